@@ -1,0 +1,244 @@
+"""Python binding of the MI355X johansen-null-eigenspectra engine.
+
+A thin ctypes layer over ``lib/libjne.so`` (the C ABI of ``include/jne.h``).
+The compute runs only in the HIP kernels of that library: if the library is
+missing or no GPU is visible, the calls raise -- there is no CPU fallback.
+
+Mirrors the reference's interface for the hot path:
+
+* ``calculate_eigenvalues(dim, steps, seed, model)`` -- the per-walk operator
+  of ``src/johansen_statistics.rs:59-85`` (returns |rho| sorted descending);
+* ``Engine.eigenvalues_by_seed`` -- the batched form used in place of the
+  per-seed rayon loop of ``src/data_storage/parallel_compute.rs:14-41``;
+* ``JohansenModel`` -- the model enum of ``src/johansen_models.rs:6-141``.
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libjne.so")
+
+JNE_OK = 0
+JNE_ERR_INVALID_ARG = -1
+JNE_ERR_HIP = -2
+JNE_ERR_UNSUPPORTED = -3
+JNE_ERR_NOMEM = -4
+JNE_ERR_NUMERIC = -5
+JNE_ERR_IO = -6
+
+PREC_FP64 = 0
+PREC_FP32_KAHAN = 1
+PREC_MFMA_FP16X2 = 2
+PREC_MFMA_BF16X2 = 3
+PREC_MFMA_BF16X3 = 4
+PRECISIONS = {
+    "fp64": PREC_FP64,
+    "fp32_kahan": PREC_FP32_KAHAN,
+    "mfma_fp16x2": PREC_MFMA_FP16X2,
+    "mfma_bf16x2": PREC_MFMA_BF16X2,
+    "mfma_bf16x3": PREC_MFMA_BF16X3,
+}
+
+MAX_DIM = 12
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+class JneError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[jne error {code}] {message}")
+        self.code = code
+
+
+def load_library() -> ctypes.CDLL:
+    """Load lib/libjne.so; raises if it has not been built."""
+    global _lib
+    with _lib_lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} not built (run __graft_entry__.build() or make -C {_HERE})")
+        L = ctypes.CDLL(LIB_PATH)
+        c_void_pp = ctypes.POINTER(ctypes.c_void_p)
+        L.jne_create.argtypes = [ctypes.c_int, c_void_pp]
+        L.jne_create.restype = ctypes.c_int
+        L.jne_destroy.argtypes = [ctypes.c_void_p]
+        L.jne_destroy.restype = None
+        L.jne_last_error.argtypes = [ctypes.c_void_p]
+        L.jne_last_error.restype = ctypes.c_char_p
+        L.jne_num_eigenvalues.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.jne_num_eigenvalues.restype = ctypes.c_int
+        L.jne_eigenvalues_by_seed.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                              ctypes.c_uint32, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
+                                              ctypes.c_void_p]
+        L.jne_eigenvalues_by_seed.restype = ctypes.c_int
+        L.jne_eigenvalues_by_seed_device.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                                     ctypes.c_uint32, ctypes.c_void_p, ctypes.c_size_t,
+                                                     ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        L.jne_eigenvalues_by_seed_device.restype = ctypes.c_int
+        L.jne_eigenvalues_from_normals.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                                   ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p]
+        L.jne_eigenvalues_from_normals.restype = ctypes.c_int
+        L.jne_walk_normals.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32,
+                                       ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
+        L.jne_walk_normals.restype = ctypes.c_int
+        L.jne_synchronize.argtypes = [ctypes.c_void_p]
+        L.jne_synchronize.restype = ctypes.c_int
+        _lib = L
+        return L
+
+
+def num_eigenvalues(model: int, dim: int) -> int:
+    """dim+1 for models 1 and 3, else dim (src/data_storage/thread_manager.rs:40-44)."""
+    n = load_library().jne_num_eigenvalues(int(model), int(dim))
+    if n < 0:
+        raise JneError(n, f"invalid model {model} or dim {dim}")
+    return n
+
+
+class JohansenModel(enum.IntEnum):
+    """Model enum of src/johansen_models.rs:6-141 (ids 0..4, default 2)."""
+
+    NoInterceptNoTrend = 0
+    InterceptNoTrendWithInterceptInCoint = 1
+    InterceptNoTrendUnrestrictedIntercept = 2
+    InterceptTrendUnrestrictedInterceptRestrictedTrend = 3
+    InterceptTrendUnrestrictedBoth = 4
+
+    def to_number(self) -> int:
+        return int(self)
+
+    @classmethod
+    def from_number(cls, n: int):
+        try:
+            return cls(int(n))
+        except ValueError:
+            return None
+
+    def description(self) -> str:
+        return _DESCRIPTIONS[int(self)]
+
+    def has_intercept(self) -> bool:
+        return self != JohansenModel.NoInterceptNoTrend
+
+    def has_trend(self) -> bool:
+        return int(self) in (3, 4)
+
+    def intercept_fully_explained_by_cointegration(self) -> bool:
+        return int(self) == 1
+
+    def trend_fully_explained_by_cointegration(self) -> bool:
+        return int(self) == 3
+
+    @classmethod
+    def all_models(cls):
+        return [cls(i) for i in range(5)]
+
+    @classmethod
+    def default(cls):
+        return cls.InterceptNoTrendUnrestrictedIntercept
+
+    def __str__(self) -> str:  # Display impl, src/johansen_models.rs:137-141
+        return f"Model {int(self)}: {self.description()}"
+
+
+_DESCRIPTIONS = (
+    "No intercept, no trend",
+    "Intercept, no trend, intercept in cointegration",
+    "Intercept, no trend, intercept not fully explained by cointegration",
+    "Intercept, trend, trend in cointegration",
+    "Intercept, trend, intercept and trend not fully explained by cointegration",
+)
+
+
+class Engine:
+    """One jne_ctx: a HIP device + stream; calls on one Engine are serialised."""
+
+    def __init__(self, device: int = 0):
+        self._lib = load_library()
+        ctx = ctypes.c_void_p()
+        rc = self._lib.jne_create(int(device), ctypes.byref(ctx))
+        if rc != JNE_OK:
+            raise JneError(rc, self._lib.jne_last_error(None).decode())
+        self._ctx = ctx
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self._lib.jne_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc: int):
+        if rc != JNE_OK:
+            raise JneError(rc, self._lib.jne_last_error(self._ctx).decode())
+
+    def eigenvalues_by_seed(self, model: int, dim: int, steps: int, seeds, rng_stream: int = 0,
+                            precision: int = PREC_FP64) -> np.ndarray:
+        s = np.ascontiguousarray(seeds, dtype=np.uint32)
+        nf = num_eigenvalues(model, dim)
+        out = np.empty((len(s), nf), dtype=np.float64)
+        self._check(self._lib.jne_eigenvalues_by_seed(self._ctx, int(model), int(dim), int(steps), int(rng_stream),
+                                                      s.ctypes.data, len(s), int(precision), out.ctypes.data))
+        return out
+
+    def eigenvalues_by_seed_device(self, model: int, dim: int, steps: int, d_seeds: int, n: int, d_out: int,
+                                   rng_stream: int = 0, precision: int = PREC_FP64, hip_stream: int = 0):
+        """Device pointers (ints); enqueues on hip_stream (0 = the context's stream)."""
+        self._check(self._lib.jne_eigenvalues_by_seed_device(
+            self._ctx, int(model), int(dim), int(steps), int(rng_stream), ctypes.c_void_p(d_seeds), int(n),
+            int(precision), ctypes.c_void_p(d_out), ctypes.c_void_p(hip_stream or None)))
+
+    def eigenvalues_from_normals(self, model: int, z, precision: int = PREC_FP64) -> np.ndarray:
+        """z: (n, steps, dim) float64 N(0,1) draws."""
+        z = np.ascontiguousarray(z, dtype=np.float64)
+        if z.ndim == 2:
+            z = z[None]
+        n, steps, dim = z.shape
+        nf = num_eigenvalues(model, dim)
+        out = np.empty((n, nf), dtype=np.float64)
+        self._check(self._lib.jne_eigenvalues_from_normals(self._ctx, int(model), int(dim), int(steps),
+                                                           z.ctypes.data, n, int(precision), out.ctypes.data))
+        return out
+
+    def walk_normals(self, dim: int, steps: int, seeds, rng_stream: int = 0) -> np.ndarray:
+        s = np.ascontiguousarray(seeds, dtype=np.uint32)
+        out = np.empty((len(s), steps, dim), dtype=np.float32)
+        self._check(self._lib.jne_walk_normals(self._ctx, int(dim), int(steps), int(rng_stream), s.ctypes.data,
+                                               len(s), out.ctypes.data))
+        return out
+
+    def synchronize(self):
+        self._check(self._lib.jne_synchronize(self._ctx))
+
+
+_default_engine = None
+
+
+def default_engine() -> Engine:
+    global _default_engine
+    if _default_engine is None:
+        _default_engine = Engine(0)
+    return _default_engine
+
+
+def calculate_eigenvalues(dim: int, steps: int, seed: int, model) -> list:
+    """Per-walk operator of src/johansen_statistics.rs:59-85 on the device."""
+    return default_engine().eigenvalues_by_seed(int(model), dim, steps, [seed])[0].tolist()

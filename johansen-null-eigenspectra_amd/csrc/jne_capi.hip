@@ -1,0 +1,277 @@
+// jne_capi.hip -- the C ABI (include/jne.h): contexts, argument checks,
+// chunked launches of the walk kernels, host/device copies.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+
+#include "../../include/jne.h"
+#include "jne_internal.h"
+#include "jne_rng.h"
+
+namespace jne {
+
+// One thread per Philox call of one walk: writes normals 4q..4q+3.
+__global__ __launch_bounds__(256) void walk_normals_kernel(const uint32_t* __restrict__ seeds, uint32_t rng_stream,
+                                                           int d, uint32_t T, uint32_t n, uint64_t calls_per_walk,
+                                                           float* __restrict__ out) {
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t walk = gid / calls_per_walk;
+  if (walk >= n) return;
+  const uint64_t q = gid - walk * calls_per_walk;
+  float v[4];
+  normals4(q, seeds[walk], rng_stream, v);
+  const uint64_t total = (uint64_t)T * d;
+  float* o = out + walk * total;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint64_t idx = 4 * q + k;
+    if (idx < total) o[idx] = v[k];
+  }
+}
+
+hipError_t launch_walk_normals(int d, const uint32_t* seeds, uint32_t rng_stream, uint32_t T, uint32_t n,
+                               float* out, hipStream_t st) {
+  const uint64_t calls = ((uint64_t)T * d + 3) / 4;
+  const uint64_t threads = calls * n;
+  const uint64_t grid = (threads + 255) / 256;
+  hipLaunchKernelGGL(walk_normals_kernel, dim3((unsigned)grid), dim3(256), 0, st, seeds, rng_stream, d, T, n, calls,
+                     out);
+  return hipGetLastError();
+}
+
+}  // namespace jne
+
+struct jne_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  std::string err;
+  // device scratch: per-walk moments of one chunk
+  double* d_acc = nullptr;
+  size_t acc_bytes = 0;
+  // device staging for the host-pointer entry points
+  void* d_in = nullptr;
+  size_t in_bytes = 0;
+  void* d_out = nullptr;
+  size_t out_bytes = 0;
+};
+
+namespace {
+
+thread_local std::string g_create_err;
+constexpr size_t kChunkWalks = size_t(1) << 18;
+
+int set_err(jne_ctx* ctx, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  ctx->err = buf;
+  return code;
+}
+
+#define JNE_HIP_CHECK(ctx, expr)                                                                   \
+  do {                                                                                             \
+    hipError_t e_ = (expr);                                                                        \
+    if (e_ != hipSuccess) return set_err((ctx), JNE_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+int ensure(jne_ctx* ctx, void** ptr, size_t* have, size_t need) {
+  if (*have >= need) return JNE_OK;
+  if (*ptr) {
+    (void)hipFree(*ptr);
+    *ptr = nullptr;
+    *have = 0;
+  }
+  hipError_t e = hipMalloc(ptr, need);
+  if (e != hipSuccess) {
+    *ptr = nullptr;
+    return set_err(ctx, JNE_ERR_NOMEM, "hipMalloc(%zu bytes) failed: %s", need, hipGetErrorString(e));
+  }
+  *have = need;
+  return JNE_OK;
+}
+
+int check_args(jne_ctx* ctx, int model, int dim, uint32_t steps, int precision) {
+  if (model < 0 || model > 4) return set_err(ctx, JNE_ERR_INVALID_ARG, "invalid model %d (expected 0..4)", model);
+  if (dim < 1 || dim > JNE_MAX_DIM)
+    return set_err(ctx, JNE_ERR_INVALID_ARG, "invalid dim %d (expected 1..%d)", dim, JNE_MAX_DIM);
+  if (steps < 2 || steps > (1u << 30))
+    return set_err(ctx, JNE_ERR_INVALID_ARG, "invalid steps %u (expected 2..2^30)", steps);
+  if (precision != JNE_PREC_FP64)
+    return set_err(ctx, JNE_ERR_UNSUPPORTED, "precision %d is not available in this build", precision);
+  return JNE_OK;
+}
+
+// Enqueue the walk path for n walks (device pointers) on stream st.
+int run_walks(jne_ctx* ctx, int model, int dim, uint32_t steps, uint32_t rng_stream, const uint32_t* d_seeds,
+              const double* d_z, size_t n, double* d_out, hipStream_t st) {
+  const int nf = jne::num_eigenvalues(model, dim);
+  const size_t mc = (size_t)jne::moment_count(dim);
+  const size_t chunk = std::min(n, kChunkWalks);
+  int rc = ensure(ctx, (void**)&ctx->d_acc, &ctx->acc_bytes, chunk * mc * sizeof(double));
+  if (rc) return rc;
+  for (size_t off = 0; off < n; off += chunk) {
+    const uint32_t cn = (uint32_t)std::min(chunk, n - off);
+    const uint32_t* s = d_seeds ? d_seeds + off : nullptr;
+    const double* z = d_z ? d_z + off * (size_t)steps * dim : nullptr;
+    JNE_HIP_CHECK(ctx, jne::launch_walk_moments_fp64(dim, s, z, rng_stream, steps, cn, ctx->d_acc, cn, st));
+    JNE_HIP_CHECK(ctx, jne::launch_walk_eigen_fp64(model, dim, ctx->d_acc, cn, steps, cn, d_out + off * nf, st));
+  }
+  return JNE_OK;
+}
+
+int finish_host_out(jne_ctx* ctx, double* out, size_t count) {
+  JNE_HIP_CHECK(ctx, hipMemcpyAsync(out, ctx->d_out, count * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  JNE_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  for (size_t i = 0; i < count; ++i)
+    if (isnan(out[i]))
+      return set_err(ctx, JNE_ERR_NUMERIC, "walk %zu: S11 is not positive definite", i);
+  return JNE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int jne_create(int device, jne_ctx** out) {
+  if (!out) {
+    g_create_err = "jne_create: out is NULL";
+    return JNE_ERR_INVALID_ARG;
+  }
+  *out = nullptr;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count <= 0) {
+    g_create_err = std::string("jne_create: no HIP device: ") + hipGetErrorString(e);
+    return JNE_ERR_HIP;
+  }
+  if (device < 0 || device >= count) {
+    g_create_err = "jne_create: device index out of range";
+    return JNE_ERR_INVALID_ARG;
+  }
+  jne_ctx* ctx = new jne_ctx();
+  ctx->device = device;
+  if ((e = hipSetDevice(device)) != hipSuccess || (e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess) {
+    g_create_err = std::string("jne_create: ") + hipGetErrorString(e);
+    delete ctx;
+    return JNE_ERR_HIP;
+  }
+  *out = ctx;
+  return JNE_OK;
+}
+
+void jne_destroy(jne_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->d_acc) (void)hipFree(ctx->d_acc);
+  if (ctx->d_in) (void)hipFree(ctx->d_in);
+  if (ctx->d_out) (void)hipFree(ctx->d_out);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* jne_last_error(const jne_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
+
+int jne_num_eigenvalues(int model, int dim) {
+  if (model < 0 || model > 4 || dim < 1 || dim > 255) return JNE_ERR_INVALID_ARG;
+  return jne::num_eigenvalues(model, dim);
+}
+
+int jne_eigenvalues_by_seed(jne_ctx* ctx, int model, int dim, uint32_t steps, uint32_t rng_stream,
+                            const uint32_t* seeds, size_t n, int precision, double* out) {
+  if (!ctx) return JNE_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->err.clear();
+  int rc = check_args(ctx, model, dim, steps, precision);
+  if (rc) return rc;
+  if (n == 0) return JNE_OK;
+  if (!seeds || !out) return set_err(ctx, JNE_ERR_INVALID_ARG, "seeds/out must not be NULL");
+  JNE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  const size_t nf = (size_t)jne::num_eigenvalues(model, dim);
+  if ((rc = ensure(ctx, &ctx->d_in, &ctx->in_bytes, n * sizeof(uint32_t)))) return rc;
+  if ((rc = ensure(ctx, &ctx->d_out, &ctx->out_bytes, n * nf * sizeof(double)))) return rc;
+  JNE_HIP_CHECK(ctx, hipMemcpyAsync(ctx->d_in, seeds, n * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+  if ((rc = run_walks(ctx, model, dim, steps, rng_stream, (const uint32_t*)ctx->d_in, nullptr, n,
+                      (double*)ctx->d_out, ctx->stream)))
+    return rc;
+  return finish_host_out(ctx, out, n * nf);
+}
+
+int jne_eigenvalues_by_seed_device(jne_ctx* ctx, int model, int dim, uint32_t steps, uint32_t rng_stream,
+                                   const uint32_t* d_seeds, size_t n, int precision, double* d_out,
+                                   void* hip_stream) {
+  if (!ctx) return JNE_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->err.clear();
+  int rc = check_args(ctx, model, dim, steps, precision);
+  if (rc) return rc;
+  if (n == 0) return JNE_OK;
+  if (!d_seeds || !d_out) return set_err(ctx, JNE_ERR_INVALID_ARG, "d_seeds/d_out must not be NULL");
+  JNE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = hip_stream ? (hipStream_t)hip_stream : ctx->stream;
+  return run_walks(ctx, model, dim, steps, rng_stream, d_seeds, nullptr, n, d_out, st);
+}
+
+int jne_eigenvalues_from_normals(jne_ctx* ctx, int model, int dim, uint32_t steps, const double* z, size_t n,
+                                 int precision, double* out) {
+  if (!ctx) return JNE_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->err.clear();
+  int rc = check_args(ctx, model, dim, steps, precision);
+  if (rc) return rc;
+  if (n == 0) return JNE_OK;
+  if (!z || !out) return set_err(ctx, JNE_ERR_INVALID_ARG, "z/out must not be NULL");
+  JNE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  const size_t nf = (size_t)jne::num_eigenvalues(model, dim);
+  const size_t zcount = n * (size_t)steps * dim;
+  if ((rc = ensure(ctx, &ctx->d_in, &ctx->in_bytes, zcount * sizeof(double)))) return rc;
+  if ((rc = ensure(ctx, &ctx->d_out, &ctx->out_bytes, n * nf * sizeof(double)))) return rc;
+  JNE_HIP_CHECK(ctx, hipMemcpyAsync(ctx->d_in, z, zcount * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  if ((rc = run_walks(ctx, model, dim, steps, 0, nullptr, (const double*)ctx->d_in, n, (double*)ctx->d_out,
+                      ctx->stream)))
+    return rc;
+  return finish_host_out(ctx, out, n * nf);
+}
+
+int jne_walk_normals(jne_ctx* ctx, int dim, uint32_t steps, uint32_t rng_stream, const uint32_t* seeds, size_t n,
+                     float* out) {
+  if (!ctx) return JNE_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->err.clear();
+  if (dim < 1 || dim > 255) return set_err(ctx, JNE_ERR_INVALID_ARG, "invalid dim %d", dim);
+  if (steps < 1 || steps > (1u << 30)) return set_err(ctx, JNE_ERR_INVALID_ARG, "invalid steps %u", steps);
+  if (n == 0) return JNE_OK;
+  if (!seeds || !out) return set_err(ctx, JNE_ERR_INVALID_ARG, "seeds/out must not be NULL");
+  if (((uint64_t)steps * dim + 3) / 4 * n > (uint64_t)INT32_MAX * 256)
+    return set_err(ctx, JNE_ERR_INVALID_ARG, "jne_walk_normals: request too large");
+  JNE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  const size_t total = n * (size_t)steps * dim;
+  int rc;
+  if ((rc = ensure(ctx, &ctx->d_in, &ctx->in_bytes, n * sizeof(uint32_t)))) return rc;
+  if ((rc = ensure(ctx, &ctx->d_out, &ctx->out_bytes, total * sizeof(float)))) return rc;
+  JNE_HIP_CHECK(ctx, hipMemcpyAsync(ctx->d_in, seeds, n * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+  JNE_HIP_CHECK(ctx, jne::launch_walk_normals(dim, (const uint32_t*)ctx->d_in, rng_stream, steps, (uint32_t)n,
+                                             (float*)ctx->d_out, ctx->stream));
+  JNE_HIP_CHECK(ctx, hipMemcpyAsync(out, ctx->d_out, total * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  JNE_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return JNE_OK;
+}
+
+int jne_synchronize(jne_ctx* ctx) {
+  if (!ctx) return JNE_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  JNE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  JNE_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return JNE_OK;
+}
+
+}  // extern "C"

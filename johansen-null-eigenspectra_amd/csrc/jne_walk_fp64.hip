@@ -1,0 +1,478 @@
+// jne_walk_fp64.hip -- FP64 walk path: one-pass moments + FP64 eigen epilogue.
+//
+// Replaces, per walk, the reference's
+//   brownian_motion_matrix      src/rng_matrix.rs:57-141
+//   calculate_eigenvalues       src/johansen_statistics.rs:59-85
+//   construct_f_matrix          src/johansen_statistics.rs:102-197
+//   sum_of_outer_products       src/matrix_utils.rs:67-85
+//   GeneralizedEigen (dggev)    src/johansen_statistics.rs:35-45
+// without materialising B, dB or F.  Kernel 1 streams the walk once and keeps
+// the moments of x_t = [b_t, 1, u_t, u_t^2, e_t] (b_t = B_t before step t,
+// e_t = B_{t+1} - B_t, u_t = (t+1)/T); kernel 2 forms S11 and S01 of the
+// requested model from those moments (demeaning and detrending are Schur
+// complements on the {1, u} columns) and solves the eigenproblem.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <type_traits>
+
+#include "jne_internal.h"
+#include "jne_rng.h"
+
+namespace jne {
+
+// ---------------------------------------------------------------------------
+// Kernel 1: moments.
+//
+// A walk is owned by a group of L lanes.  Every lane keeps the whole
+// state vector b (dim doubles, in a lane-specific rotated order) and
+// accumulates its share of the Gram entries:
+//   rows k < R of  sum e b^T               (R = DP/L, DP = dim padded to L)
+//   rows k < R, cyclic offsets 0..DP/2 of  sum b b^T
+//   rows k < R of  sum u^p b, sum u^p e    (p = 0, 1, 2)
+// Lane r's row k is component (r*R + k) mod DP, so the L lanes cover every
+// entry; register indices are compile-time constants and the per-lane
+// rotation is applied when normals are read from LDS.
+//
+// Normals are produced in blocks of S = 4L steps: lane r of a group makes the
+// dim Philox calls holding steps t0+4r .. t0+4r+3 and writes them to LDS.
+// ---------------------------------------------------------------------------
+template <int D, bool FROM_NORMALS>
+__global__ __launch_bounds__(64) void walk_moments_fp64(const uint32_t* __restrict__ seeds,
+                                                        const double* __restrict__ zin, uint32_t rng_stream,
+                                                        uint32_t T, uint32_t n_walks, double* __restrict__ acc,
+                                                        size_t ld) {
+  constexpr int L = WalkShape<D>::L;
+  constexpr int DP = WalkShape<D>::DP;
+  constexpr int R = DP / L;
+  constexpr int S = 4 * L;
+  constexpr int GPW = 64 / L;
+  constexpr int H = DP / 2;
+  using ZT = typename std::conditional<FROM_NORMALS, double, float>::type;
+  __shared__ ZT zs[GPW][S * D];
+
+  const int lane = threadIdx.x;
+  const int g = lane / L;
+  const int r = lane - g * L;
+  const bool in_group = g < GPW;
+  const int gs = in_group ? g : 0;
+  const size_t walk = (size_t)blockIdx.x * GPW + gs;
+  const bool valid = in_group && walk < n_walks;
+  const size_t wc = walk < n_walks ? walk : (size_t)(n_walks - 1);
+  uint32_t seed = 0;
+  if constexpr (!FROM_NORMALS) seed = seeds[wc];
+
+  // rotated component of local index m
+  const int rot = r * R;
+
+  double b[DP];
+  double geb[R][DP];
+  double gbb[R][H + 1];
+  double mom[6][R];
+#pragma unroll
+  for (int m = 0; m < DP; ++m) b[m] = 0.0;
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+#pragma unroll
+    for (int m = 0; m < DP; ++m) geb[k][m] = 0.0;
+#pragma unroll
+    for (int m = 0; m <= H; ++m) gbb[k][m] = 0.0;
+#pragma unroll
+    for (int p = 0; p < 6; ++p) mom[p][k] = 0.0;
+  }
+
+  // LDS offsets of this lane's rotated components (-1 for padding)
+  int coff[DP];
+#pragma unroll
+  for (int m = 0; m < DP; ++m) {
+    int c = rot + m;
+    if (c >= DP) c -= DP;
+    coff[m] = c < D ? c : -1;
+  }
+
+  const double dt = 1.0 / (double)T;
+  const double sqrt_dt = sqrt(dt);
+
+  for (uint32_t t0 = 0; t0 < T; t0 += S) {
+    if (in_group) {
+      if constexpr (!FROM_NORMALS) {
+        const uint64_t qbase = (uint64_t)t0 * D / 4 + (uint64_t)r * D;
+#pragma unroll 2
+        for (int j = 0; j < D; ++j) {
+          float v[4];
+          normals4(qbase + j, seed, rng_stream, v);
+#pragma unroll
+          for (int k = 0; k < 4; ++k) zs[gs][4 * r * D + 4 * j + k] = v[k];
+        }
+      } else {
+        const double* zw = zin + wc * (size_t)T * D;
+        for (int k = 0; k < 4 * D; ++k) {
+          const uint32_t t = t0 + 4 * r + k / D;
+          zs[gs][4 * r * D + k] = t < T ? zw[(size_t)t * D + (k % D)] : 0.0;
+        }
+      }
+    }
+    __syncthreads();
+    const int nsteps = (T - t0) < (uint32_t)S ? (int)(T - t0) : S;
+    for (int s = 0; s < nsteps; ++s) {
+      const double u = (double)(t0 + s + 1) * dt;
+      const double u2 = u * u;
+      double bn[DP];
+#pragma unroll
+      for (int m = 0; m < DP; ++m) {
+        const double z = coff[m] >= 0 ? (double)zs[gs][s * D + coff[m]] : 0.0;
+        // B_{t+1} = B_t + z*sqrt(dt) (rng_matrix.rs:138-140)
+        bn[m] = b[m] + z * sqrt_dt;
+      }
+      double e[R];
+#pragma unroll
+      for (int k = 0; k < R; ++k) e[k] = bn[k] - b[k];  // dbm = bm_current - bm_previous
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+#pragma unroll
+        for (int m = 0; m < DP; ++m) geb[k][m] = fma(e[k], b[m], geb[k][m]);
+#pragma unroll
+        for (int dl = 0; dl <= H; ++dl) gbb[k][dl] = fma(b[k], b[(k + dl) % DP], gbb[k][dl]);
+        mom[0][k] += b[k];
+        mom[1][k] = fma(u, b[k], mom[1][k]);
+        mom[2][k] = fma(u2, b[k], mom[2][k]);
+        mom[3][k] += e[k];
+        mom[4][k] = fma(u, e[k], mom[4][k]);
+        mom[5][k] = fma(u2, e[k], mom[5][k]);
+      }
+#pragma unroll
+      for (int m = 0; m < DP; ++m) b[m] = bn[m];
+    }
+    __syncthreads();
+  }
+
+  if (!valid) return;
+  // canonical layout, entry-major: [0, D^2) sum b b^T (full), [D^2, 2D^2)
+  // sum e b^T (row e, col b), then 6 moment vectors of length D.
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const int ci = coff[k];
+    if (ci < 0) continue;
+#pragma unroll
+    for (int m = 0; m < DP; ++m) {
+      const int cj = coff[m];
+      if (cj >= 0) acc[(size_t)(D * D + ci * D + cj) * ld + walk] = geb[k][m];
+    }
+#pragma unroll
+    for (int dl = 0; dl <= H; ++dl) {
+      const int cj = coff[(k + dl) % DP];
+      if (cj >= 0) {
+        acc[(size_t)(ci * D + cj) * ld + walk] = gbb[k][dl];
+        acc[(size_t)(cj * D + ci) * ld + walk] = gbb[k][dl];
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < 6; ++p) acc[(size_t)(2 * D * D + p * D + ci) * ld + walk] = mom[p][k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Kernel 2: per-walk model algebra + generalized eigen solve (one thread per
+// walk).  F_t = X_t - P Z_t with P = (sum X Z^T)(sum Z Z^T)^-1 applied to the
+// projected rows of X:
+//   model 0: X = b,            no Z
+//   model 1: X = [b; 1],       no Z
+//   model 2: X = [b_0..b_{d-2}; y], Z = [1], y = u - 1/2 not projected
+//   model 3: X = [b; y],       Z = [1], y not projected
+//   model 4: X = [b_0..b_{d-2}; u^2], Z = [1; u], all rows projected
+// S11 = dt (XX - sum_{projected} XZ ZZ^-1 ZX),  S01raw = eX - eZ ZZ^-1 ZX.
+// The eigenvalues of S11^-1 S10 S01 are those of M = W^T W, W = L^-1 S10
+// (S11 = L L^T), plus nF - dim structural zeros (models 1 and 3).
+// ---------------------------------------------------------------------------
+template <int N>
+struct SymReg {
+  // upper triangle (i <= j) of an N x N symmetric matrix in registers
+  double v[N * (N + 1) / 2];
+  __device__ __forceinline__ static constexpr int idx(int i, int j) {
+    return i <= j ? i * N - i * (i - 1) / 2 + (j - i) : j * N - j * (j - 1) / 2 + (i - j);
+  }
+  __device__ __forceinline__ double& operator()(int i, int j) { return v[idx(i, j)]; }
+};
+
+template <int N>
+__device__ __forceinline__ void jacobi_rotate(SymReg<N>& a, int p, int q) {
+  const double apq = a(p, q);
+  if (apq == 0.0) return;
+  const double app = a(p, p), aqq = a(q, q);
+  const double theta = (aqq - app) / (2.0 * apq);
+  double t = 1.0 / (fabs(theta) + sqrt(fma(theta, theta, 1.0)));
+  if (theta < 0.0) t = -t;
+  const double c = 1.0 / sqrt(fma(t, t, 1.0));
+  const double s = t * c;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    if (k == p || k == q) continue;
+    const double akp = a(k, p), akq = a(k, q);
+    a(k, p) = c * akp - s * akq;
+    a(k, q) = s * akp + c * akq;
+  }
+  a(p, p) = app - t * apq;
+  a(q, q) = aqq + t * apq;
+  a(p, q) = 0.0;
+}
+
+template <int N>
+__device__ __forceinline__ void jacobi_eigvals(SymReg<N>& a) {
+  for (int sweep = 0; sweep < 40; ++sweep) {
+    double off = 0.0, dia = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      dia = fma(a(i, i), a(i, i), dia);
+#pragma unroll
+      for (int j = i + 1; j < N; ++j) off = fma(a(i, j), a(i, j), off);
+    }
+    if (!(off > 1e-34 * dia)) break;
+#pragma unroll
+    for (int p = 0; p < N - 1; ++p) {
+#pragma unroll
+      for (int q = p + 1; q < N; ++q) jacobi_rotate<N>(a, p, q);
+    }
+  }
+}
+
+template <int MODEL, int D>
+__global__ __launch_bounds__(64) void walk_eigen_fp64(const double* __restrict__ acc, size_t ld, uint32_t T,
+                                                      uint32_t n_walks, double* __restrict__ out) {
+  constexpr int NX = (MODEL == 1 || MODEL == 3) ? D + 1 : D;  // nF
+  constexpr int NB = (MODEL == 2 || MODEL == 4) ? D - 1 : D;  // b rows in X
+  constexpr int NZ = MODEL == 4 ? 2 : ((MODEL == 2 || MODEL == 3) ? 1 : 0);
+  __shared__ double wsh[NX * D][64];
+
+  const int tid = threadIdx.x;
+  const size_t w = (size_t)blockIdx.x * 64 + tid;
+  const bool valid = w < n_walks;
+  const size_t wc = valid ? w : (size_t)(n_walks - 1);
+  auto A = [&](int e) -> double { return acc[(size_t)e * ld + wc]; };
+  auto GBB = [&](int i, int j) { return A(i * D + j); };
+  auto GEB = [&](int k, int j) { return A(D * D + k * D + j); };
+  auto MB = [&](int p, int i) { return A(2 * D * D + p * D + i); };
+  auto ME = [&](int p, int k) { return A(2 * D * D + (3 + p) * D + k); };
+
+  const double Tn = (double)T;
+  const double dt = 1.0 / Tn;
+  // closed forms of sum_t u_t^p, u_t = (t+1)/T, t = 0..T-1
+  const double c1 = (Tn + 1.0) * 0.5;
+  const double c2 = (Tn + 1.0) * (2.0 * Tn + 1.0) / (6.0 * Tn);
+  const double c3 = (Tn + 1.0) * (Tn + 1.0) / (4.0 * Tn);
+  const double c4 = (Tn + 1.0) * (2.0 * Tn + 1.0) * (3.0 * Tn * Tn + 3.0 * Tn - 1.0) / (30.0 * Tn * Tn * Tn);
+
+  // XZ row x, column z (z = 0: 1, z = 1: u)
+  auto XZ = [&](int x, int z) -> double {
+    if (x < NB) return MB(z, x);
+    if constexpr (MODEL == 2 || MODEL == 3) return c1 - 0.5 * Tn;  // sum y
+    else if constexpr (MODEL == 4) return z == 0 ? c2 : c3;        // sum u^2, sum u^3
+    else return 0.0;
+  };
+  auto projected = [&](int x) -> bool {
+    if constexpr (MODEL == 4) return true;
+    else if constexpr (MODEL == 2 || MODEL == 3) return x < NB;
+    else return false;
+  };
+  auto XX = [&](int i, int j) -> double {
+    if (i < NB && j < NB) return GBB(i, j);
+    if (i >= NB && j >= NB) {
+      if constexpr (MODEL == 1) return Tn;
+      else if constexpr (MODEL == 2 || MODEL == 3) return c2 - c1 + 0.25 * Tn;  // sum (u - 1/2)^2
+      else return c4;
+    }
+    const int bi = i < NB ? i : j;
+    if constexpr (MODEL == 1) return MB(0, bi);
+    else if constexpr (MODEL == 2 || MODEL == 3) return MB(1, bi) - 0.5 * MB(0, bi);  // sum b y
+    else return MB(2, bi);                                                            // sum b u^2
+  };
+  auto EX = [&](int k, int j) -> double {
+    if (j < NB) return GEB(k, j);
+    if constexpr (MODEL == 1) return ME(0, k);
+    else if constexpr (MODEL == 2 || MODEL == 3) return ME(1, k) - 0.5 * ME(0, k);
+    else return ME(2, k);
+  };
+
+  // ZZ^-1 (closed form): model 4: [[T, c1], [c1, c2]]^-1, det = (T^2 - 1)/12
+  double zi00 = 0.0, zi01 = 0.0, zi11 = 0.0;
+  if constexpr (NZ == 1) zi00 = dt;
+  if constexpr (NZ == 2) {
+    const double det = (Tn * Tn - 1.0) / 12.0;
+    zi00 = c2 / det;
+    zi01 = -c1 / det;
+    zi11 = Tn / det;
+  }
+  // P_x = XZ_x ZZ^-1 for every X row
+  double P0[NX], P1[NX];
+#pragma unroll
+  for (int x = 0; x < NX; ++x) {
+    P0[x] = 0.0;
+    P1[x] = 0.0;
+    if constexpr (NZ == 1) P0[x] = XZ(x, 0) * zi00;
+    if constexpr (NZ == 2) {
+      const double a0 = XZ(x, 0), a1 = XZ(x, 1);
+      P0[x] = a0 * zi00 + a1 * zi01;
+      P1[x] = a0 * zi01 + a1 * zi11;
+    }
+  }
+  auto proj = [&](int i, int j) -> double {  // XZ_i ZZ^-1 ZX_j with projection mask
+    if constexpr (NZ == 0) return 0.0;
+    else {
+      if (projected(i)) return NZ == 2 ? P0[i] * XZ(j, 0) + P1[i] * XZ(j, 1) : P0[i] * XZ(j, 0);
+      if (projected(j)) return NZ == 2 ? P0[j] * XZ(i, 0) + P1[j] * XZ(i, 1) : P0[j] * XZ(i, 0);
+      return 0.0;
+    }
+  };
+
+  // S11 (lower triangle in SymReg form) and its Cholesky factor in place
+  SymReg<NX> Lf;
+  bool pd = true;
+#pragma unroll
+  for (int i = 0; i < NX; ++i) {
+#pragma unroll
+    for (int j = 0; j <= i; ++j) Lf(i, j) = (XX(i, j) - proj(i, j)) * dt;
+  }
+#pragma unroll
+  for (int j = 0; j < NX; ++j) {
+    double s = Lf(j, j);
+#pragma unroll
+    for (int k = 0; k < j; ++k) s -= Lf(j, k) * Lf(j, k);
+    pd = pd && (s > 0.0);
+    const double ljj = sqrt(s > 0.0 ? s : 1.0);
+    const double inv = 1.0 / ljj;
+    Lf(j, j) = ljj;
+#pragma unroll
+    for (int i = j + 1; i < NX; ++i) {
+      double v = Lf(i, j);
+#pragma unroll
+      for (int k = 0; k < j; ++k) v -= Lf(i, k) * Lf(j, k);
+      Lf(i, j) = v * inv;
+    }
+  }
+
+  // W column k = L^-1 (row k of S01raw), stored to LDS
+#pragma unroll 1
+  for (int k = 0; k < D; ++k) {
+    double wv[NX];
+#pragma unroll
+    for (int i = 0; i < NX; ++i) {
+      double v = EX(k, i);
+      if constexpr (NZ > 0) {
+        if (projected(i)) {
+          const double ez0 = ME(0, k);
+          v -= NZ == 2 ? P0[i] * ez0 + P1[i] * ME(1, k) : P0[i] * ez0;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < i; ++j) v -= Lf(i, j) * wv[j];
+      wv[i] = v / Lf(i, i);
+      wsh[i * D + k][tid] = wv[i];
+    }
+  }
+
+  // M = W^T W (D x D)
+  SymReg<D> M;
+#pragma unroll
+  for (int k = 0; k < D * (D + 1) / 2; ++k) M.v[k] = 0.0;
+#pragma unroll 1
+  for (int i = 0; i < NX; ++i) {
+    double row[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) row[k] = wsh[i * D + k][tid];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+#pragma unroll
+      for (int l = k; l < D; ++l) M(k, l) = fma(row[k], row[l], M(k, l));
+    }
+  }
+
+  jacobi_eigvals<D>(M);
+
+  double ev[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) ev[k] = fabs(M(k, k));
+  // sort descending (odd-even transposition, fixed network)
+#pragma unroll
+  for (int pass = 0; pass < D; ++pass) {
+#pragma unroll
+    for (int k = pass & 1; k + 1 < D; k += 2) {
+      const double hi = fmax(ev[k], ev[k + 1]);
+      const double lo = fmin(ev[k], ev[k + 1]);
+      ev[k] = hi;
+      ev[k + 1] = lo;
+    }
+  }
+  if (!valid) return;
+  double* o = out + w * NX;
+  const double nan = __longlong_as_double(0x7FF8000000000000LL);
+#pragma unroll
+  for (int k = 0; k < D; ++k) o[k] = pd ? ev[k] : nan;
+  if constexpr (NX > D) o[D] = pd ? 0.0 : nan;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+template <int D>
+static hipError_t launch_moments_d(const uint32_t* seeds, const double* zin, uint32_t rng_stream, uint32_t T,
+                                   uint32_t n, double* acc, size_t ld, hipStream_t st) {
+  constexpr int GPW = 64 / WalkShape<D>::L;
+  const unsigned grid = (unsigned)((n + GPW - 1) / GPW);
+  if (zin != nullptr)
+    hipLaunchKernelGGL((walk_moments_fp64<D, true>), dim3(grid), dim3(64), 0, st, seeds, zin, rng_stream, T, n,
+                       acc, ld);
+  else
+    hipLaunchKernelGGL((walk_moments_fp64<D, false>), dim3(grid), dim3(64), 0, st, seeds, zin, rng_stream, T, n,
+                       acc, ld);
+  return hipGetLastError();
+}
+
+template <int MODEL, int D>
+static hipError_t launch_eigen_md(const double* acc, size_t ld, uint32_t T, uint32_t n, double* out,
+                                  hipStream_t st) {
+  const unsigned grid = (unsigned)((n + 63) / 64);
+  hipLaunchKernelGGL((walk_eigen_fp64<MODEL, D>), dim3(grid), dim3(64), 0, st, acc, ld, T, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_walk_moments_fp64(int d, const uint32_t* seeds, const double* zin, uint32_t rng_stream,
+                                    uint32_t T, uint32_t n, double* acc, size_t ld, hipStream_t st) {
+  switch (d) {
+#define JNE_CASE(DD) \
+  case DD:           \
+    return launch_moments_d<DD>(seeds, zin, rng_stream, T, n, acc, ld, st);
+    JNE_CASE(1) JNE_CASE(2) JNE_CASE(3) JNE_CASE(4) JNE_CASE(5) JNE_CASE(6)
+    JNE_CASE(7) JNE_CASE(8) JNE_CASE(9) JNE_CASE(10) JNE_CASE(11) JNE_CASE(12)
+#undef JNE_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+template <int MODEL>
+static hipError_t launch_eigen_m(int d, const double* acc, size_t ld, uint32_t T, uint32_t n, double* out,
+                                 hipStream_t st) {
+  switch (d) {
+#define JNE_CASE(DD) \
+  case DD:           \
+    return launch_eigen_md<MODEL, DD>(acc, ld, T, n, out, st);
+    JNE_CASE(1) JNE_CASE(2) JNE_CASE(3) JNE_CASE(4) JNE_CASE(5) JNE_CASE(6)
+    JNE_CASE(7) JNE_CASE(8) JNE_CASE(9) JNE_CASE(10) JNE_CASE(11) JNE_CASE(12)
+#undef JNE_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_walk_eigen_fp64(int model, int d, const double* acc, size_t ld, uint32_t T, uint32_t n,
+                                  double* out, hipStream_t st) {
+  switch (model) {
+    case 0: return launch_eigen_m<0>(d, acc, ld, T, n, out, st);
+    case 1: return launch_eigen_m<1>(d, acc, ld, T, n, out, st);
+    case 2: return launch_eigen_m<2>(d, acc, ld, T, n, out, st);
+    case 3: return launch_eigen_m<3>(d, acc, ld, T, n, out, st);
+    case 4: return launch_eigen_m<4>(d, acc, ld, T, n, out, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace jne

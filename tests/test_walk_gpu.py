@@ -1,0 +1,107 @@
+"""GPU parity of the FP64 walk path against the oracle, through the C ABI.
+
+Tolerances: eigenvalues within 1e-9 relative of the oracle (reference
+algorithm + LAPACK dggev on the same increments); the structural zero
+eigenvalue of models 1 and 3 is compared with an absolute floor of
+1e-9 * rho_max.  Normals are compared bit for bit.
+"""
+import numpy as np
+import pytest
+
+from conftest import assert_eigs_close
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-9
+
+
+def test_normals_bit_exact(engine, O):
+    for dim, steps in [(1, 7), (3, 33), (5, 101), (12, 257)]:
+        seeds = np.array([1, 2, 1000, 0xFFFFFFFF], dtype=np.uint32)
+        got = engine.walk_normals(dim, steps, seeds)
+        for k, s in enumerate(seeds):
+            ref = O.walk_normals(int(s), 0, dim, steps)
+            assert np.array_equal(got[k].view(np.uint32), ref.view(np.uint32)), (dim, steps, s)
+    # a non-zero rng stream gives a different, equally exact stream
+    got = engine.walk_normals(4, 50, [7], rng_stream=12345)
+    assert np.array_equal(got[0].view(np.uint32), O.walk_normals(7, 12345, 4, 50).view(np.uint32))
+    assert not np.array_equal(got[0], O.walk_normals(7, 0, 4, 50))
+
+
+@pytest.mark.parametrize("model", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("dim", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12])
+def test_by_seed_matches_oracle_small(engine, O, model, dim):
+    steps = 97 if dim % 2 else 128  # includes steps not a multiple of the 4L block
+    seeds = np.arange(1, 6, dtype=np.uint32)
+    got = engine.eigenvalues_by_seed(model, dim, steps, seeds)
+    assert got.shape == (5, O.num_eigenvalues(model, dim))
+    ref = np.stack([O.walk_eigenvalues_by_seed(model, dim, steps, int(s)) for s in seeds])
+    assert_eigs_close(got, ref, RTOL, f"model {model} dim {dim} steps {steps}")
+
+
+@pytest.mark.parametrize("model,dim,steps", [(2, 5, 5000), (4, 12, 10000), (3, 12, 2000), (1, 7, 3001)])
+def test_by_seed_matches_oracle_large(engine, O, model, dim, steps):
+    seeds = np.array([1, 17, 999], dtype=np.uint32)
+    got = engine.eigenvalues_by_seed(model, dim, steps, seeds)
+    ref = np.stack([O.walk_eigenvalues_by_seed(model, dim, steps, int(s)) for s in seeds])
+    assert_eigs_close(got, ref, RTOL, f"model {model} dim {dim} steps {steps}")
+
+
+def test_from_normals_equals_by_seed(engine, O):
+    model, dim, steps = 4, 6, 300
+    seeds = np.arange(1, 40, dtype=np.uint32)
+    z = engine.walk_normals(dim, steps, seeds).astype(np.float64)
+    a = engine.eigenvalues_from_normals(model, z)
+    b = engine.eigenvalues_by_seed(model, dim, steps, seeds)
+    assert np.array_equal(a, b)
+
+
+def test_from_normals_arbitrary_increments(engine, O):
+    rng = np.random.default_rng(5)
+    for model in range(5):
+        z = rng.standard_normal((3, 211, 4))
+        got = engine.eigenvalues_from_normals(model, z)
+        ref = np.stack([O.walk_eigenvalues_lapack(model, z[k]) for k in range(3)])
+        assert_eigs_close(got, ref, RTOL, f"model {model}")
+
+
+def test_batch_invariance(engine):
+    """A walk's result depends only on its seed, not on its slot in the batch."""
+    model, dim, steps = 4, 12, 500
+    seeds = np.arange(1, 301, dtype=np.uint32)
+    full = engine.eigenvalues_by_seed(model, dim, steps, seeds)
+    perm = np.random.default_rng(0).permutation(len(seeds))
+    shuffled = engine.eigenvalues_by_seed(model, dim, steps, seeds[perm])
+    assert np.array_equal(full[perm], shuffled)
+    alone = engine.eigenvalues_by_seed(model, dim, steps, seeds[123:124])
+    assert np.array_equal(alone[0], full[123])
+
+
+def test_ragged_batch_sizes(engine, O):
+    model, dim, steps = 2, 11, 64
+    for n in [1, 15, 16, 17, 63, 65]:
+        seeds = np.arange(100, 100 + n, dtype=np.uint32)
+        got = engine.eigenvalues_by_seed(model, dim, steps, seeds)
+        ref = O.walks_by_seed_c(model, dim, steps, seeds)
+        assert_eigs_close(got, ref, RTOL, f"n={n}")
+
+
+def test_empty_and_invalid(engine, jne):
+    out = engine.eigenvalues_by_seed(0, 2, 10, np.zeros(0, dtype=np.uint32))
+    assert out.shape == (0, 2)
+    with pytest.raises(jne.JneError) as e:
+        engine.eigenvalues_by_seed(5, 2, 10, [1])
+    assert e.value.code == jne.JNE_ERR_INVALID_ARG and "invalid model" in str(e.value)
+    with pytest.raises(jne.JneError):
+        engine.eigenvalues_by_seed(0, 13, 10, [1])
+    with pytest.raises(jne.JneError):
+        engine.eigenvalues_by_seed(0, 2, 1, [1])
+    # the context stays usable after an error
+    assert engine.eigenvalues_by_seed(0, 1, 10, [1]).shape == (1, 1)
+
+
+def test_calculate_eigenvalues_mirror(jne, O):
+    got = jne.calculate_eigenvalues(3, 200, 42, jne.JohansenModel.InterceptTrendUnrestrictedBoth)
+    ref = O.walk_eigenvalues_by_seed(4, 3, 200, 42)
+    assert_eigs_close(np.array(got), ref, RTOL)
+    assert got == sorted(got, reverse=True)
