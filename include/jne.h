@@ -104,6 +104,17 @@ int jne_walk_normals(jne_ctx *ctx, int dim, uint32_t steps, uint32_t rng_stream,
 /* Block until all work enqueued on the context's stream has finished. */
 int jne_synchronize(jne_ctx *ctx);
 
+/* Kernel timing (no counterpart in the reference, which has no profiling;
+ * used by bench.py for the roofline figure).  While enabled, HIP events are
+ * recorded on the launch stream around every launch of the walk kernels.
+ * jne_profile_enable clears earlier records; jne_profile_read synchronises
+ * on the recorded events and returns the summed duration (ms), the number of
+ * launches and the number of walks they processed. */
+#define JNE_KERNEL_MOMENTS 0 /* one-pass moment (Gram) accumulation */
+#define JNE_KERNEL_EIGEN 1   /* model algebra + eigen solve */
+int jne_profile_enable(jne_ctx *ctx, int enable);
+int jne_profile_read(jne_ctx *ctx, int kernel, double *total_ms, uint64_t *launches, uint64_t *walks);
+
 #ifdef __cplusplus
 }
 #endif

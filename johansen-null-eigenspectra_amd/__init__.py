@@ -47,6 +47,9 @@ PRECISIONS = {
 
 MAX_DIM = 12
 
+KERNEL_MOMENTS = 0
+KERNEL_EIGEN = 1
+
 _lib = None
 _lib_lock = threading.Lock()
 
@@ -91,6 +94,11 @@ def load_library() -> ctypes.CDLL:
         L.jne_walk_normals.restype = ctypes.c_int
         L.jne_synchronize.argtypes = [ctypes.c_void_p]
         L.jne_synchronize.restype = ctypes.c_int
+        L.jne_profile_enable.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.jne_profile_enable.restype = ctypes.c_int
+        L.jne_profile_read.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]
+        L.jne_profile_read.restype = ctypes.c_int
         _lib = L
         return L
 
@@ -227,6 +235,18 @@ class Engine:
 
     def synchronize(self):
         self._check(self._lib.jne_synchronize(self._ctx))
+
+    def profile_enable(self, enable: bool = True):
+        self._check(self._lib.jne_profile_enable(self._ctx, 1 if enable else 0))
+
+    def profile_read(self, kernel: int):
+        """(total_ms, launches, walks) of kernel KERNEL_MOMENTS or KERNEL_EIGEN."""
+        ms = ctypes.c_double()
+        nl = ctypes.c_uint64()
+        nw = ctypes.c_uint64()
+        self._check(self._lib.jne_profile_read(self._ctx, int(kernel), ctypes.byref(ms), ctypes.byref(nl),
+                                               ctypes.byref(nw)))
+        return ms.value, nl.value, nw.value
 
 
 _default_engine = None

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/jne.h"
 #include "jne_internal.h"
@@ -60,6 +61,15 @@ struct jne_ctx {
   size_t in_bytes = 0;
   void* d_out = nullptr;
   size_t out_bytes = 0;
+  // kernel timing (jne_profile_*)
+  struct TimedLaunch {
+    int kernel;
+    hipEvent_t start, stop;
+    uint64_t walks;
+  };
+  bool profiling = false;
+  std::vector<TimedLaunch> timed;
+  std::vector<hipEvent_t> event_pool;
 };
 
 namespace {
@@ -110,6 +120,43 @@ int check_args(jne_ctx* ctx, int model, int dim, uint32_t steps, int precision) 
   return JNE_OK;
 }
 
+hipEvent_t pool_event(jne_ctx* ctx) {
+  if (!ctx->event_pool.empty()) {
+    hipEvent_t e = ctx->event_pool.back();
+    ctx->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+void release_timed(jne_ctx* ctx) {
+  for (auto& t : ctx->timed) {
+    ctx->event_pool.push_back(t.start);
+    ctx->event_pool.push_back(t.stop);
+  }
+  ctx->timed.clear();
+}
+
+// Records events around `launch` on st when profiling is enabled.
+template <class F>
+hipError_t timed_launch(jne_ctx* ctx, int kernel, uint64_t walks, hipStream_t st, F&& launch) {
+  if (!ctx->profiling) return launch();
+  hipEvent_t a = pool_event(ctx), b = pool_event(ctx);
+  if (!a || !b) return hipErrorOutOfMemory;
+  hipError_t e = hipEventRecord(a, st);
+  if (e == hipSuccess) e = launch();
+  if (e == hipSuccess) e = hipEventRecord(b, st);
+  if (e != hipSuccess) {
+    ctx->event_pool.push_back(a);
+    ctx->event_pool.push_back(b);
+    return e;
+  }
+  ctx->timed.push_back({kernel, a, b, walks});
+  return hipSuccess;
+}
+
 // Enqueue the walk path for n walks (device pointers) on stream st.
 int run_walks(jne_ctx* ctx, int model, int dim, uint32_t steps, uint32_t rng_stream, const uint32_t* d_seeds,
               const double* d_z, size_t n, double* d_out, hipStream_t st) {
@@ -122,8 +169,12 @@ int run_walks(jne_ctx* ctx, int model, int dim, uint32_t steps, uint32_t rng_str
     const uint32_t cn = (uint32_t)std::min(chunk, n - off);
     const uint32_t* s = d_seeds ? d_seeds + off : nullptr;
     const double* z = d_z ? d_z + off * (size_t)steps * dim : nullptr;
-    JNE_HIP_CHECK(ctx, jne::launch_walk_moments_fp64(dim, s, z, rng_stream, steps, cn, ctx->d_acc, cn, st));
-    JNE_HIP_CHECK(ctx, jne::launch_walk_eigen_fp64(model, dim, ctx->d_acc, cn, steps, cn, d_out + off * nf, st));
+    JNE_HIP_CHECK(ctx, timed_launch(ctx, JNE_KERNEL_MOMENTS, cn, st, [&] {
+                    return jne::launch_walk_moments_fp64(dim, s, z, rng_stream, steps, cn, ctx->d_acc, cn, st);
+                  }));
+    JNE_HIP_CHECK(ctx, timed_launch(ctx, JNE_KERNEL_EIGEN, cn, st, [&] {
+                    return jne::launch_walk_eigen_fp64(model, dim, ctx->d_acc, cn, steps, cn, d_out + off * nf, st);
+                  }));
   }
   return JNE_OK;
 }
@@ -172,6 +223,8 @@ void jne_destroy(jne_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  release_timed(ctx);
+  for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
   if (ctx->d_acc) (void)hipFree(ctx->d_acc);
   if (ctx->d_in) (void)hipFree(ctx->d_in);
   if (ctx->d_out) (void)hipFree(ctx->d_out);
@@ -271,6 +324,39 @@ int jne_synchronize(jne_ctx* ctx) {
   std::lock_guard<std::mutex> lock(ctx->mu);
   JNE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   JNE_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return JNE_OK;
+}
+
+int jne_profile_enable(jne_ctx* ctx, int enable) {
+  if (!ctx) return JNE_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  JNE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  for (auto& t : ctx->timed) {
+    (void)hipEventSynchronize(t.stop);
+  }
+  release_timed(ctx);
+  ctx->profiling = enable != 0;
+  return JNE_OK;
+}
+
+int jne_profile_read(jne_ctx* ctx, int kernel, double* total_ms, uint64_t* launches, uint64_t* walks) {
+  if (!ctx || !total_ms || !launches || !walks) return JNE_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  JNE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  double ms = 0.0;
+  uint64_t nl = 0, nw = 0;
+  for (auto& t : ctx->timed) {
+    if (t.kernel != kernel) continue;
+    JNE_HIP_CHECK(ctx, hipEventSynchronize(t.stop));
+    float e = 0.0f;
+    JNE_HIP_CHECK(ctx, hipEventElapsedTime(&e, t.start, t.stop));
+    ms += e;
+    ++nl;
+    nw += t.walks;
+  }
+  *total_ms = ms;
+  *launches = nl;
+  *walks = nw;
   return JNE_OK;
 }
 
