@@ -100,6 +100,21 @@ def test_empty_and_invalid(engine, jne):
     assert engine.eigenvalues_by_seed(0, 1, 10, [1]).shape == (1, 1)
 
 
+def test_golden_fixtures(engine):
+    """Committed fixtures (oracle: reference algorithm + LAPACK dggev)."""
+    import json
+    import os
+
+    fx = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "walk_eigenvalues.json")))
+    for c in fx["cases"]:
+        got = engine.eigenvalues_by_seed(c["model"], c["dim"], c["steps"], [c["seed"]])[0]
+        assert_eigs_close(got, np.array(c["eigenvalues"]), RTOL, str(c))
+    sn = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "stream_normals.json")))
+    for c in sn["cases"]:
+        z = engine.walk_normals(c["dim"], c["steps"], [c["seed"]], rng_stream=c["rng_stream"])[0]
+        assert z.ravel().view(np.uint32).tolist() == c["bits"]
+
+
 def test_calculate_eigenvalues_mirror(jne, O):
     got = jne.calculate_eigenvalues(3, 200, 42, jne.JohansenModel.InterceptTrendUnrestrictedBoth)
     ref = O.walk_eigenvalues_by_seed(4, 3, 200, 42)
