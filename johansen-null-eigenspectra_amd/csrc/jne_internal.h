@@ -9,15 +9,42 @@ namespace jne {
 
 constexpr int kMaxDim = 12;
 
-// Lanes per walk (L) and dim padded to a multiple of L (DP) for the FP64
-// moment kernel.  Chosen so that one lane holds <= ~90 double accumulators.
-constexpr int lanes_for_dim(int d) {
-  return d <= 5 ? 1 : d <= 8 ? 2 : d == 9 ? 3 : d == 10 ? 5 : 4;
+// Per-dim shape of the FP64 moment kernel: lanes per walk (L) and target
+// waves per SIMD (the register cap).  Measured on MI355X (model 4, T = 10^4).
+struct Fp64Shape {
+  int lanes, waves;
+};
+#ifndef JNE_FP64_SHAPES
+#define JNE_FP64_SHAPES \
+  {1, 2}, {1, 2}, {1, 2}, {1, 2}, {5, 4}, {2, 3}, {7, 4}, {2, 2}, {9, 4}, {5, 3}, {4, 2}, {4, 2}
+#endif
+constexpr Fp64Shape kFp64Shapes[12] = {JNE_FP64_SHAPES};
+constexpr int lanes_for_dim(int d) { return kFp64Shapes[d - 1].lanes; }
+constexpr int waves_for_dim(int d) { return kFp64Shapes[d - 1].waves; }
+constexpr int gcd_int(int a, int b) { return b == 0 ? a : gcd_int(b, a % b); }
+
+// Steps per round of the FP64 moment kernel: a multiple of 4/gcd(d,4) (so a
+// round is a whole number of 4-normal Philox calls), at least 4, preferring a
+// call count divisible by L while one wave's double-buffered LDS stays <= 24 KiB.
+constexpr int steps_per_round(int d, int L) {
+  const int s0 = 4 / gcd_int(d, 4);
+  const int first = ((4 + s0 - 1) / s0) * s0;
+  for (int s = first; s <= 8 * s0; s += s0) {
+    const int calls = s * d / 4;
+    const int lds = 2 * (64 / L) * s * d * 8;
+    if (calls % L == 0 && lds <= 24 * 1024) return s;
+  }
+  return first;
 }
+
 template <int D>
 struct WalkShape {
-  static constexpr int L = lanes_for_dim(D);
-  static constexpr int DP = ((D + L - 1) / L) * L;
+  static constexpr int L = lanes_for_dim(D);          // lanes per walk
+  static constexpr int DP = ((D + L - 1) / L) * L;     // dim padded to L
+  static constexpr int GPW = 64 / L;                   // walks per wave
+  static constexpr int S = steps_per_round(D, L);      // steps per round
+  static constexpr int C = S * D / 4;                  // Philox calls per round
+  static_assert((S * D) % 4 == 0, "a round must be whole Philox calls");
 };
 
 // Number of per-walk moment doubles written by the FP64 moment kernel:

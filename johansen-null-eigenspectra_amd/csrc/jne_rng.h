@@ -36,14 +36,14 @@ __device__ __forceinline__ u32x4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_
                                                uint32_t k0, uint32_t k1) {
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
-    const uint32_t lo0 = kPhiloxM0 * c0;
-    const uint32_t hi0 = __umulhi(kPhiloxM0, c0);
-    const uint32_t lo1 = kPhiloxM1 * c2;
-    const uint32_t hi1 = __umulhi(kPhiloxM1, c2);
-    c0 = hi1 ^ c1 ^ k0;
-    c1 = lo1;
-    c2 = hi0 ^ c3 ^ k1;
-    c3 = lo0;
+    // one v_mad_u64_u32 per product gives both halves
+    const uint64_t p0 = (uint64_t)kPhiloxM0 * c0;
+    const uint64_t p1 = (uint64_t)kPhiloxM1 * c2;
+    // three-input XOR (truth table 0x96) in one v_bitop3_b32
+    c0 = __builtin_amdgcn_bitop3_b32((uint32_t)(p1 >> 32), c1, k0, 0x96);
+    c1 = (uint32_t)p1;
+    c2 = __builtin_amdgcn_bitop3_b32((uint32_t)(p0 >> 32), c3, k1, 0x96);
+    c3 = (uint32_t)p0;
     k0 += kPhiloxW0;
     k1 += kPhiloxW1;
   }

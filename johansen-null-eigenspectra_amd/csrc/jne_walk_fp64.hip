@@ -19,6 +19,14 @@
 #include "jne_internal.h"
 #include "jne_rng.h"
 
+// Tuning: unroll the steps of a round / the Philox calls of a round.
+#ifndef JNE_FP64_UNROLL_STEPS
+#define JNE_FP64_UNROLL_STEPS 0
+#endif
+#ifndef JNE_FP64_UNROLL_CALLS
+#define JNE_FP64_UNROLL_CALLS 0
+#endif
+
 namespace jne {
 
 // ---------------------------------------------------------------------------
@@ -31,25 +39,32 @@ namespace jne {
 //   rows k < R, cyclic offsets 0..DP/2 of  sum b b^T
 //   rows k < R of  sum u^p b, sum u^p e    (p = 0, 1, 2)
 // Lane r's row k is component (r*R + k) mod DP, so the L lanes cover every
-// entry; register indices are compile-time constants and the per-lane
-// rotation is applied when normals are read from LDS.
+// entry; register indices are compile-time constants.
 //
-// Normals are produced in blocks of S = 4L steps: lane r of a group makes the
-// dim Philox calls holding steps t0+4r .. t0+4r+3 and writes them to LDS.
+// The walk advances in rounds of S steps.  The S*dim normals of a round are
+// S*dim/4 Philox calls, dealt round-robin to the group's lanes; each normal
+// is scaled by sqrt(1/T) once and stored to LDS as a double, twice per step
+// row (row = [w_0..w_{dim-1}, 0.., w_0..w_{dim-1}, 0..], 2*DP entries) so that
+// a lane's rotated read of DP entries is one base address plus immediates.
+// The walk index is the fastest LDS dimension (odd pitch) so that the lanes
+// of a wave read from different banks.
 // ---------------------------------------------------------------------------
 template <int D, bool FROM_NORMALS>
-__global__ __launch_bounds__(64) void walk_moments_fp64(const uint32_t* __restrict__ seeds,
-                                                        const double* __restrict__ zin, uint32_t rng_stream,
-                                                        uint32_t T, uint32_t n_walks, double* __restrict__ acc,
-                                                        size_t ld) {
-  constexpr int L = WalkShape<D>::L;
-  constexpr int DP = WalkShape<D>::DP;
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(waves_for_dim(D), waves_for_dim(D))))
+void walk_moments_fp64(const uint32_t* __restrict__ seeds, const double* __restrict__ zin, uint32_t rng_stream,
+                       uint32_t T, uint32_t n_walks, double* __restrict__ acc, size_t ld) {
+  using Sh = WalkShape<D>;
+  constexpr int L = Sh::L;
+  constexpr int DP = Sh::DP;
   constexpr int R = DP / L;
-  constexpr int S = 4 * L;
-  constexpr int GPW = 64 / L;
   constexpr int H = DP / 2;
-  using ZT = typename std::conditional<FROM_NORMALS, double, float>::type;
-  __shared__ ZT zs[GPW][S * D];
+  constexpr int GPW = Sh::GPW;
+  constexpr int S = Sh::S;
+  constexpr int C = Sh::C;
+  constexpr int IT = (C + L - 1) / L;
+  constexpr int RW = 2 * DP;        // LDS row width per step
+  constexpr int GP = GPW | 1;       // odd walk pitch: a wave's reads spread over the banks
+  __shared__ double ws[S][RW][GP];
 
   const int lane = threadIdx.x;
   const int g = lane / L;
@@ -61,9 +76,17 @@ __global__ __launch_bounds__(64) void walk_moments_fp64(const uint32_t* __restri
   const size_t wc = walk < n_walks ? walk : (size_t)(n_walks - 1);
   uint32_t seed = 0;
   if constexpr (!FROM_NORMALS) seed = seeds[wc];
-
-  // rotated component of local index m
+  const double* zw = FROM_NORMALS ? zin + wc * (size_t)T * D : nullptr;
   const int rot = r * R;
+
+  if constexpr (DP != D) {  // zero the padding columns once
+    if (in_group)
+      for (int s = 0; s < S; ++s)
+        for (int i = D + r; i < DP; i += L) {
+          ws[s][i][gs] = 0.0;
+          ws[s][DP + i][gs] = 0.0;
+        }
+  }
 
   double b[DP];
   double geb[R][DP];
@@ -81,52 +104,68 @@ __global__ __launch_bounds__(64) void walk_moments_fp64(const uint32_t* __restri
     for (int p = 0; p < 6; ++p) mom[p][k] = 0.0;
   }
 
-  // LDS offsets of this lane's rotated components (-1 for padding)
-  int coff[DP];
-#pragma unroll
-  for (int m = 0; m < DP; ++m) {
-    int c = rot + m;
-    if (c >= DP) c -= DP;
-    coff[m] = c < D ? c : -1;
-  }
-
+  // src/johansen_statistics.rs:70 and src/rng_matrix.rs:137: sqrt(1/steps)
   const double dt = 1.0 / (double)T;
   const double sqrt_dt = sqrt(dt);
 
   for (uint32_t t0 = 0; t0 < T; t0 += S) {
+    // produce: this round's increments
     if (in_group) {
-      if constexpr (!FROM_NORMALS) {
-        const uint64_t qbase = (uint64_t)t0 * D / 4 + (uint64_t)r * D;
-#pragma unroll 2
-        for (int j = 0; j < D; ++j) {
-          float v[4];
-          normals4(qbase + j, seed, rng_stream, v);
+      const uint64_t q0 = (uint64_t)t0 * D / 4;
+#if JNE_FP64_UNROLL_CALLS
 #pragma unroll
-          for (int k = 0; k < 4; ++k) zs[gs][4 * r * D + 4 * j + k] = v[k];
-        }
-      } else {
-        const double* zw = zin + wc * (size_t)T * D;
-        for (int k = 0; k < 4 * D; ++k) {
-          const uint32_t t = t0 + 4 * r + k / D;
-          zs[gs][4 * r * D + k] = t < T ? zw[(size_t)t * D + (k % D)] : 0.0;
+#else
+#pragma unroll 1
+#endif
+      for (int it = 0; it < IT; ++it) {
+        const int c = it * L + r;
+        if (IT * L == C || c < C) {
+          double wv[4];
+          if constexpr (!FROM_NORMALS) {
+            float v[4];
+            normals4(q0 + c, seed, rng_stream, v);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) wv[j] = (double)v[j] * sqrt_dt;
+          } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const int nl = 4 * c + j;
+              const uint32_t t = t0 + nl / D;
+              wv[j] = t < T ? zw[(size_t)t * D + (nl % D)] * sqrt_dt : 0.0;
+            }
+          }
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const int nl = 4 * c + j;
+            const int s = nl / D, i = nl % D;
+            ws[s][i][gs] = wv[j];
+            ws[s][DP + i][gs] = wv[j];
+          }
         }
       }
     }
     __syncthreads();
-    const int nsteps = (T - t0) < (uint32_t)S ? (int)(T - t0) : S;
-    for (int s = 0; s < nsteps; ++s) {
-      const double u = (double)(t0 + s + 1) * dt;
+    // consume: advance the walk S steps
+    const int ns = (T - t0) < (uint32_t)S ? (int)(T - t0) : S;
+#if JNE_FP64_UNROLL_STEPS
+#pragma unroll
+#else
+#pragma unroll 1
+#endif
+    for (int s = 0; s < S; ++s) {
+      if (s >= ns) break;
+      const double* w = &ws[s][rot][gs];
+      const uint32_t t = t0 + s;
+      const double u = (double)(t + 1) * dt;
       const double u2 = u * u;
-      double bn[DP];
+      // only the lane's own rows need B_{t+1} before the Gram update; the
+      // other components are advanced in place afterwards
+      double bnk[R], e[R];
 #pragma unroll
-      for (int m = 0; m < DP; ++m) {
-        const double z = coff[m] >= 0 ? (double)zs[gs][s * D + coff[m]] : 0.0;
-        // B_{t+1} = B_t + z*sqrt(dt) (rng_matrix.rs:138-140)
-        bn[m] = b[m] + z * sqrt_dt;
+      for (int k = 0; k < R; ++k) {
+        bnk[k] = b[k] + w[k * GP];  // B_{t+1} = B_t + z*sqrt(dt) (rng_matrix.rs:138-140)
+        e[k] = bnk[k] - b[k];  // dbm = bm_current - bm_previous (johansen_statistics.rs:80-82)
       }
-      double e[R];
-#pragma unroll
-      for (int k = 0; k < R; ++k) e[k] = bn[k] - b[k];  // dbm = bm_current - bm_previous
 #pragma unroll
       for (int k = 0; k < R; ++k) {
 #pragma unroll
@@ -141,7 +180,9 @@ __global__ __launch_bounds__(64) void walk_moments_fp64(const uint32_t* __restri
         mom[5][k] = fma(u2, e[k], mom[5][k]);
       }
 #pragma unroll
-      for (int m = 0; m < DP; ++m) b[m] = bn[m];
+      for (int k = 0; k < R; ++k) b[k] = bnk[k];
+#pragma unroll
+      for (int m = R; m < DP; ++m) b[m] = b[m] + w[m * GP];
     }
     __syncthreads();
   }
@@ -151,17 +192,17 @@ __global__ __launch_bounds__(64) void walk_moments_fp64(const uint32_t* __restri
   // sum e b^T (row e, col b), then 6 moment vectors of length D.
 #pragma unroll
   for (int k = 0; k < R; ++k) {
-    const int ci = coff[k];
-    if (ci < 0) continue;
+    const int ci = (rot + k) % DP;
+    if (ci >= D) continue;
 #pragma unroll
     for (int m = 0; m < DP; ++m) {
-      const int cj = coff[m];
-      if (cj >= 0) acc[(size_t)(D * D + ci * D + cj) * ld + walk] = geb[k][m];
+      const int cj = (rot + m) % DP;
+      if (cj < D) acc[(size_t)(D * D + ci * D + cj) * ld + walk] = geb[k][m];
     }
 #pragma unroll
     for (int dl = 0; dl <= H; ++dl) {
-      const int cj = coff[(k + dl) % DP];
-      if (cj >= 0) {
+      const int cj = (rot + k + dl) % DP;
+      if (cj < D) {
         acc[(size_t)(ci * D + cj) * ld + walk] = gbb[k][dl];
         acc[(size_t)(cj * D + ci) * ld + walk] = gbb[k][dl];
       }
@@ -415,7 +456,7 @@ __global__ __launch_bounds__(64) void walk_eigen_fp64(const double* __restrict__
 template <int D>
 static hipError_t launch_moments_d(const uint32_t* seeds, const double* zin, uint32_t rng_stream, uint32_t T,
                                    uint32_t n, double* acc, size_t ld, hipStream_t st) {
-  constexpr int GPW = 64 / WalkShape<D>::L;
+  constexpr int GPW = WalkShape<D>::GPW;
   const unsigned grid = (unsigned)((n + GPW - 1) / GPW);
   if (zin != nullptr)
     hipLaunchKernelGGL((walk_moments_fp64<D, true>), dim3(grid), dim3(64), 0, st, seeds, zin, rng_stream, T, n,
