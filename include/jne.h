@@ -42,6 +42,9 @@ extern "C" {
 #define JNE_PREC_MFMA_FP16X2 2 /* MFMA, fp16 hi+lo operands, fp32 accumulate */
 #define JNE_PREC_MFMA_BF16X2 3 /* MFMA, bf16 hi+lo operands, fp32 accumulate */
 #define JNE_PREC_MFMA_BF16X3 4 /* MFMA, bf16 hi+mid+lo operands, fp32 accumulate */
+#define JNE_PREC_MFMA_FP32 5   /* MFMA, fp32 operands, fp32 accumulate */
+/* The MFMA modes keep the state B and the per-64-step segment terms in FP64
+ * and put only within-segment corrections on the matrix cores. */
 
 #define JNE_MAX_DIM 12
 

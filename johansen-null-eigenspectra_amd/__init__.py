@@ -37,12 +37,14 @@ PREC_FP32_KAHAN = 1
 PREC_MFMA_FP16X2 = 2
 PREC_MFMA_BF16X2 = 3
 PREC_MFMA_BF16X3 = 4
+PREC_MFMA_FP32 = 5
 PRECISIONS = {
     "fp64": PREC_FP64,
     "fp32_kahan": PREC_FP32_KAHAN,
     "mfma_fp16x2": PREC_MFMA_FP16X2,
     "mfma_bf16x2": PREC_MFMA_BF16X2,
     "mfma_bf16x3": PREC_MFMA_BF16X3,
+    "mfma_fp32": PREC_MFMA_FP32,
 }
 
 MAX_DIM = 12

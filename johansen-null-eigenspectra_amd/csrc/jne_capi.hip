@@ -115,9 +115,19 @@ int check_args(jne_ctx* ctx, int model, int dim, uint32_t steps, int precision) 
     return set_err(ctx, JNE_ERR_INVALID_ARG, "invalid dim %d (expected 1..%d)", dim, JNE_MAX_DIM);
   if (steps < 2 || steps > (1u << 30))
     return set_err(ctx, JNE_ERR_INVALID_ARG, "invalid steps %u (expected 2..2^30)", steps);
-  if (precision != JNE_PREC_FP64)
-    return set_err(ctx, JNE_ERR_UNSUPPORTED, "precision %d is not available in this build", precision);
-  return JNE_OK;
+  switch (precision) {
+    case JNE_PREC_FP64:
+    case JNE_PREC_MFMA_FP16X2:
+    case JNE_PREC_MFMA_BF16X2:
+    case JNE_PREC_MFMA_BF16X3:
+    case JNE_PREC_MFMA_FP32:
+      return JNE_OK;
+    case JNE_PREC_FP32_KAHAN:
+      return set_err(ctx, JNE_ERR_UNSUPPORTED, "precision %d (FP32 Kahan VALU) is not available in this build",
+                     precision);
+    default:
+      return set_err(ctx, JNE_ERR_INVALID_ARG, "invalid precision %d", precision);
+  }
 }
 
 hipEvent_t pool_event(jne_ctx* ctx) {
@@ -159,7 +169,7 @@ hipError_t timed_launch(jne_ctx* ctx, int kernel, uint64_t walks, hipStream_t st
 
 // Enqueue the walk path for n walks (device pointers) on stream st.
 int run_walks(jne_ctx* ctx, int model, int dim, uint32_t steps, uint32_t rng_stream, const uint32_t* d_seeds,
-              const double* d_z, size_t n, double* d_out, hipStream_t st) {
+              const double* d_z, size_t n, int precision, double* d_out, hipStream_t st) {
   const int nf = jne::num_eigenvalues(model, dim);
   const size_t mc = (size_t)jne::moment_count(dim);
   const size_t chunk = std::min(n, kChunkWalks);
@@ -170,7 +180,10 @@ int run_walks(jne_ctx* ctx, int model, int dim, uint32_t steps, uint32_t rng_str
     const uint32_t* s = d_seeds ? d_seeds + off : nullptr;
     const double* z = d_z ? d_z + off * (size_t)steps * dim : nullptr;
     JNE_HIP_CHECK(ctx, timed_launch(ctx, JNE_KERNEL_MOMENTS, cn, st, [&] {
-                    return jne::launch_walk_moments_fp64(dim, s, z, rng_stream, steps, cn, ctx->d_acc, cn, st);
+                    if (precision == JNE_PREC_FP64)
+                      return jne::launch_walk_moments_fp64(dim, s, z, rng_stream, steps, cn, ctx->d_acc, cn, st);
+                    return jne::launch_walk_moments_mfma(dim, precision, s, z, rng_stream, steps, cn, ctx->d_acc,
+                                                         cn, st);
                   }));
     JNE_HIP_CHECK(ctx, timed_launch(ctx, JNE_KERNEL_EIGEN, cn, st, [&] {
                     return jne::launch_walk_eigen_fp64(model, dim, ctx->d_acc, cn, steps, cn, d_out + off * nf, st);
@@ -253,7 +266,7 @@ int jne_eigenvalues_by_seed(jne_ctx* ctx, int model, int dim, uint32_t steps, ui
   if ((rc = ensure(ctx, &ctx->d_in, &ctx->in_bytes, n * sizeof(uint32_t)))) return rc;
   if ((rc = ensure(ctx, &ctx->d_out, &ctx->out_bytes, n * nf * sizeof(double)))) return rc;
   JNE_HIP_CHECK(ctx, hipMemcpyAsync(ctx->d_in, seeds, n * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
-  if ((rc = run_walks(ctx, model, dim, steps, rng_stream, (const uint32_t*)ctx->d_in, nullptr, n,
+  if ((rc = run_walks(ctx, model, dim, steps, rng_stream, (const uint32_t*)ctx->d_in, nullptr, n, precision,
                       (double*)ctx->d_out, ctx->stream)))
     return rc;
   return finish_host_out(ctx, out, n * nf);
@@ -271,7 +284,7 @@ int jne_eigenvalues_by_seed_device(jne_ctx* ctx, int model, int dim, uint32_t st
   if (!d_seeds || !d_out) return set_err(ctx, JNE_ERR_INVALID_ARG, "d_seeds/d_out must not be NULL");
   JNE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   hipStream_t st = hip_stream ? (hipStream_t)hip_stream : ctx->stream;
-  return run_walks(ctx, model, dim, steps, rng_stream, d_seeds, nullptr, n, d_out, st);
+  return run_walks(ctx, model, dim, steps, rng_stream, d_seeds, nullptr, n, precision, d_out, st);
 }
 
 int jne_eigenvalues_from_normals(jne_ctx* ctx, int model, int dim, uint32_t steps, const double* z, size_t n,
@@ -289,7 +302,7 @@ int jne_eigenvalues_from_normals(jne_ctx* ctx, int model, int dim, uint32_t step
   if ((rc = ensure(ctx, &ctx->d_in, &ctx->in_bytes, zcount * sizeof(double)))) return rc;
   if ((rc = ensure(ctx, &ctx->d_out, &ctx->out_bytes, n * nf * sizeof(double)))) return rc;
   JNE_HIP_CHECK(ctx, hipMemcpyAsync(ctx->d_in, z, zcount * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-  if ((rc = run_walks(ctx, model, dim, steps, 0, nullptr, (const double*)ctx->d_in, n, (double*)ctx->d_out,
+  if ((rc = run_walks(ctx, model, dim, steps, 0, nullptr, (const double*)ctx->d_in, n, precision, (double*)ctx->d_out,
                       ctx->stream)))
     return rc;
   return finish_host_out(ctx, out, n * nf);

@@ -47,6 +47,39 @@ struct WalkShape {
   static_assert((S * D) % 4 == 0, "a round must be whole Philox calls");
 };
 
+// Step s and component i (n = s*D + i) of the normals n = 4c .. 4c+3 drawn by
+// Philox call c of a round.  One division where the position is created, then
+// constant strides: the loops over calls advance c by a compile-time step.
+template <int D>
+struct CallPos {
+  int s, i;  // of normal 4c
+  __device__ __forceinline__ explicit CallPos(int c) : s((4 * c) / D), i(4 * c - ((4 * c) / D) * D) {}
+  template <int DC>
+  __device__ __forceinline__ void advance() {
+    constexpr int ds = (4 * DC) / D, di = (4 * DC) % D;
+    s += ds;
+    i += di;
+    if (i >= D) {
+      i -= D;
+      ++s;
+    }
+  }
+  // position of normal 4c + j, j = 0..3
+  __device__ __forceinline__ void at(int j, int& sj, int& ij) const {
+    // D % 4 == 0: i is a multiple of 4, so i + 3 < D; otherwise up to 3 wraps (D < 4)
+    constexpr int kWraps = D % 4 == 0 ? 0 : D >= 4 ? 1 : 3;
+    sj = s;
+    ij = i + j;
+#pragma unroll
+    for (int w = 0; w < kWraps; ++w) {
+      if (ij >= D) {
+        ij -= D;
+        ++sj;
+      }
+    }
+  }
+};
+
 // Number of per-walk moment doubles written by the FP64 moment kernel:
 // sum b b^T (D*D), sum e b^T (D*D), 6 moment vectors (6*D).
 constexpr int moment_count(int d) { return 2 * d * d + 6 * d; }
@@ -55,6 +88,9 @@ constexpr int num_eigenvalues(int model, int d) { return (model == 1 || model ==
 
 hipError_t launch_walk_moments_fp64(int d, const uint32_t* seeds, const double* zin, uint32_t rng_stream,
                                     uint32_t T, uint32_t n, double* acc, size_t ld, hipStream_t st);
+hipError_t launch_walk_moments_mfma(int d, int precision, const uint32_t* seeds, const double* zin,
+                                    uint32_t rng_stream, uint32_t T, uint32_t n, double* acc, size_t ld,
+                                    hipStream_t st);
 hipError_t launch_walk_eigen_fp64(int model, int d, const double* acc, size_t ld, uint32_t T, uint32_t n,
                                   double* out, hipStream_t st);
 hipError_t launch_walk_normals(int d, const uint32_t* seeds, uint32_t rng_stream, uint32_t T, uint32_t n,

@@ -78,6 +78,7 @@ void walk_moments_fp64(const uint32_t* __restrict__ seeds, const double* __restr
   if constexpr (!FROM_NORMALS) seed = seeds[wc];
   const double* zw = FROM_NORMALS ? zin + wc * (size_t)T * D : nullptr;
   const int rot = r * R;
+  const CallPos<D> pos0(r);  // normals of this lane's first Philox call in a round
 
   if constexpr (DP != D) {  // zero the padding columns once
     if (in_group)
@@ -112,6 +113,7 @@ void walk_moments_fp64(const uint32_t* __restrict__ seeds, const double* __restr
     // produce: this round's increments
     if (in_group) {
       const uint64_t q0 = (uint64_t)t0 * D / 4;
+      CallPos<D> pos = pos0;
 #if JNE_FP64_UNROLL_CALLS
 #pragma unroll
 #else
@@ -120,6 +122,9 @@ void walk_moments_fp64(const uint32_t* __restrict__ seeds, const double* __restr
       for (int it = 0; it < IT; ++it) {
         const int c = it * L + r;
         if (IT * L == C || c < C) {
+          int sj[4], ij[4];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) pos.at(j, sj[j], ij[j]);
           double wv[4];
           if constexpr (!FROM_NORMALS) {
             float v[4];
@@ -129,19 +134,17 @@ void walk_moments_fp64(const uint32_t* __restrict__ seeds, const double* __restr
           } else {
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-              const int nl = 4 * c + j;
-              const uint32_t t = t0 + nl / D;
-              wv[j] = t < T ? zw[(size_t)t * D + (nl % D)] * sqrt_dt : 0.0;
+              const uint32_t t = t0 + sj[j];
+              wv[j] = t < T ? zw[(size_t)t * D + ij[j]] * sqrt_dt : 0.0;
             }
           }
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
-            const int nl = 4 * c + j;
-            const int s = nl / D, i = nl % D;
-            ws[s][i][gs] = wv[j];
-            ws[s][DP + i][gs] = wv[j];
+            ws[sj[j]][ij[j]][gs] = wv[j];
+            ws[sj[j]][DP + ij[j]][gs] = wv[j];
           }
         }
+        pos.template advance<L>();
       }
     }
     __syncthreads();

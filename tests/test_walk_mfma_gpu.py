@@ -1,0 +1,104 @@
+"""GPU parity of the mixed-precision (MFMA) moment pass, through the C ABI.
+
+The mixed path draws the same Philox increments as the FP64 path and keeps the
+state B and every per-segment term in FP64; only the within-segment Gram is
+accumulated on the matrix cores (FP32 accumulate, flushed to FP64 every 64
+steps).  Its eigenvalues are compared with the oracle / the FP64 path on the
+same seeds, normwise per walk:
+
+    max_i |rho_i(mixed) - rho_i(ref)| <= TOL * max_i |rho_i(ref)|
+
+TOL_LONG applies for steps >= 1000 (the production and precision-ladder
+configurations; measured worst case 2e-7 for fp16x2 / bf16x3 / fp32 and 7e-6
+for bf16x2 over 2,000-walk batches).  Short walks (steps < 1000) are dominated
+by the within-segment part; there TOL_SHORT still catches any structural
+mistake (a wrong row, a dropped tail step gives O(1) differences).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL_LONG = {"mfma_fp16x2": 1e-5, "mfma_bf16x3": 1e-5, "mfma_fp32": 1e-5, "mfma_bf16x2": 1e-4}
+TOL_SHORT = {"mfma_fp16x2": 2e-3, "mfma_bf16x3": 2e-3, "mfma_fp32": 2e-3, "mfma_bf16x2": 5e-2}
+PRECS = list(TOL_LONG)
+
+
+def normwise(got, ref):
+    got = np.asarray(got, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    assert got.shape == ref.shape
+    assert np.isfinite(got).all()
+    return np.max(np.abs(got - ref), axis=-1) / np.max(np.abs(ref), axis=-1)
+
+
+def tol_for(prec, steps):
+    return (TOL_LONG if steps >= 1000 else TOL_SHORT)[prec]
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("model", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("dim", [1, 2, 5, 7, 12])
+def test_mixed_matches_fp64_path(engine, jne, prec, model, dim):
+    # steps: < one segment, exactly one, several plus a tail
+    for steps in [33, 64, 200, 1000]:
+        seeds = np.arange(1, 33, dtype=np.uint32)
+        ref = engine.eigenvalues_by_seed(model, dim, steps, seeds)
+        got = engine.eigenvalues_by_seed(model, dim, steps, seeds, precision=jne.PRECISIONS[prec])
+        err = normwise(got, ref).max()
+        assert err <= tol_for(prec, steps), f"{prec} model {model} dim {dim} steps {steps}: {err:.2e}"
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_mixed_matches_oracle_headline(engine, jne, O, prec):
+    """Model 4, dim 12, T = 10,000 (the precision-ladder configuration)."""
+    seeds = np.array([1, 2, 3, 1000, 77777], dtype=np.uint32)
+    ref = O.walks_by_seed_c(4, 12, 10000, seeds, threads=5)
+    got = engine.eigenvalues_by_seed(4, 12, 10000, seeds, precision=jne.PRECISIONS[prec])
+    err = normwise(got, ref).max()
+    assert err <= TOL_LONG[prec], f"{prec}: {err:.2e}"
+
+
+@pytest.mark.parametrize("model,dim,steps", [(2, 5, 5000), (1, 7, 3001), (3, 12, 2047), (0, 9, 1100)])
+def test_mixed_matches_oracle_long(engine, jne, O, model, dim, steps):
+    seeds = np.array([5, 6, 7], dtype=np.uint32)
+    ref = O.walks_by_seed_c(model, dim, steps, seeds, threads=3)
+    got = engine.eigenvalues_by_seed(model, dim, steps, seeds, precision=jne.PRECISIONS["mfma_fp16x2"])
+    err = normwise(got, ref).max()
+    assert err <= TOL_LONG["mfma_fp16x2"], f"model {model} dim {dim} steps {steps}: {err:.2e}"
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_mixed_from_normals_equals_by_seed(engine, jne, prec):
+    """Float-representable increments give bit-identical results on both entry points."""
+    model, dim, steps = 4, 6, 300
+    seeds = np.arange(1, 20, dtype=np.uint32)
+    z = engine.walk_normals(dim, steps, seeds).astype(np.float64)
+    p = jne.PRECISIONS[prec]
+    assert np.array_equal(engine.eigenvalues_from_normals(model, z, precision=p),
+                          engine.eigenvalues_by_seed(model, dim, steps, seeds, precision=p))
+
+
+def test_mixed_batch_invariance(engine, jne):
+    p = jne.PRECISIONS["mfma_fp16x2"]
+    model, dim, steps = 3, 12, 777
+    seeds = np.arange(1, 201, dtype=np.uint32)
+    full = engine.eigenvalues_by_seed(model, dim, steps, seeds, precision=p)
+    perm = np.random.default_rng(1).permutation(len(seeds))
+    assert np.array_equal(full[perm], engine.eigenvalues_by_seed(model, dim, steps, seeds[perm], precision=p))
+    assert np.array_equal(full[57], engine.eigenvalues_by_seed(model, dim, steps, seeds[57:58], precision=p)[0])
+
+
+def test_mixed_deterministic_and_sorted(engine, jne):
+    p = jne.PRECISIONS["mfma_fp16x2"]
+    seeds = np.arange(1, 65, dtype=np.uint32)
+    a = engine.eigenvalues_by_seed(4, 12, 3000, seeds, precision=p)
+    b = engine.eigenvalues_by_seed(4, 12, 3000, seeds, precision=p)
+    assert np.array_equal(a, b)
+    assert (np.diff(a, axis=1) <= 0).all() and (a >= 0).all()
+
+
+def test_fp32_kahan_unsupported(engine, jne):
+    with pytest.raises(jne.JneError) as e:
+        engine.eigenvalues_by_seed(2, 3, 100, [1], precision=jne.PRECISIONS["fp32_kahan"])
+    assert e.value.code == jne.JNE_ERR_UNSUPPORTED
