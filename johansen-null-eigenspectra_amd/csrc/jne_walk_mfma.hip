@@ -4,15 +4,15 @@
 // walk_eigen_fp64), computed as a split between FP64 per-segment terms and an
 // MFMA Gram of small within-segment quantities.  With s = sqrt(1/T), z_t the
 // unscaled normals, a segment of n <= 64 steps [ts, ts + n) starting at the
-// FP64 state B0 = B_ts, k = t - ts and c_t = sum_{ts <= r < t} z_r (so
-// b_t = B0 + s c_t and e_t = s z_t):
+// FP64 state B0 = B_ts, k = t - ts and c_t = sum_{ts <= r < t} z_r - m with
+// m = eps/2 (so b_t = B0 + s m + s c_t and e_t = s z_t):
 //
-//   sum b b^T   = sum_seg [n B0 B0^T + s (B0 sig^T + sig B0^T)] + s^2 sum c c^T
-//   sum e b^T   = sum_seg [s eps B0^T]                          + s^2 sum z c^T
-//   sum u^p b   = sum_seg [B0 sum_seg u^p]                      + s   sum u^p c
-//   sum u^p e   =                                                 s   sum u^p z
+//   sum b b^T   = sum_seg [n B0' B0'^T + s (B0' sig^T + sig B0'^T)] + s^2 sum c c^T
+//   sum e b^T   = sum_seg [s eps B0'^T]                            + s^2 sum z c^T
+//   sum u^p b   = sum_seg [B0' sum_seg u^p]                        + s   sum u^p c
+//   sum u^p e   =                                                    s   sum u^p z
 //
-// (eps = sum_seg z, sig = sum_seg c).  The bracketed terms are FP64, once per
+// (eps = sum_seg z, sig = sum_seg c, B0' = B0 + s m).  The bracketed terms are FP64, once per
 // segment.  The rest comes from the Gram of the 32 rows
 //
 //   X = [z_0..z_{D-1}, 0.., c_0..c_{D-1} (from row 12), 0.., 1, k, k(k-1)/2, 0..]
@@ -37,9 +37,19 @@
 #include "jne_rng.h"
 
 // Tuning: unroll the Philox calls of a segment (independent chains for the
-// scheduler, at the cost of registers).
+// scheduler, at the cost of registers); target waves per SIMD.
 #ifndef JNE_MFMA_UNROLL_CALLS
-#define JNE_MFMA_UNROLL_CALLS 0
+#define JNE_MFMA_UNROLL_CALLS 1
+#endif
+#ifndef JNE_MFMA_WAVES
+#define JNE_MFMA_WAVES 4
+#endif
+// fp16 split: round-to-nearest hi (else truncated hi); add the lo*lo product
+#ifndef JNE_MFMA_SPLIT_ROUND
+#define JNE_MFMA_SPLIT_ROUND 1
+#endif
+#ifndef JNE_MFMA_LOLO
+#define JNE_MFMA_LOLO 0
 #endif
 
 namespace jne {
@@ -47,68 +57,75 @@ namespace jne {
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
-typedef uint32_t u32x4v __attribute__((ext_vector_type(4)));
 
 constexpr int kSegSteps = 64;  // steps per FP64 segment and per RNG round
 constexpr int kChunks = 4;     // 16-step MFMA chunks per segment
 constexpr int kRowC = 12;      // first row of the c block
 constexpr int kRowDet = 24;    // rows 24, 25, 26: 1, k, k(k-1)/2
 
+// The FP64 Gram entries the output needs, compact in LDS:
+//   column c in [12, 24) (the c block): rows 0..26  at (c - 12) * 27 + r
+//   column c < 12 (the z block):        rows 24..26 at 324 + 3 c + (r - 24)
+constexpr int kGramZ = 12 * 27;
+constexpr int kGramSize = kGramZ + 12 * 3;
+
 template <int D>
 struct MfmaShape {
   static constexpr int CALLS = kSegSteps * D / 4;  // Philox calls per segment
   static constexpr int IT = (CALLS + 63) / 64;     // call iterations per lane
-  static constexpr int ZP = kSegSteps + 4;         // LDS pitch of one component (floats)
+  static constexpr int ZP = kSegSteps + 4;         // LDS pitch of one row (floats)
+  static constexpr int ZROWS = D + 4;              // normals, then 1, k, k(k-1)/2, 0
   static constexpr int NG = (D * D + 63) / 64;     // (i, j) grid entries per lane
   static constexpr int EPS_P = 64 / D;             // lanes per component of the FP64 eps sum
-  static constexpr int ZBYTES = D * ZP * 4;
-  static constexpr int PBBYTES = 32 * 33 * 8;
-  static constexpr int SMEM = ZBYTES > PBBYTES ? ZBYTES : PBBYTES;  // normals, later the Gram
   static_assert(D <= kRowC, "dim must fit the z block");
 };
 
-// Chunk MFMA of one 16-step chunk.  x[8] are this lane's 8 row values (steps
-// 8h + k, or 2k + h for the interleaved FP32 form).
+// hi/lo splits by truncation: hi keeps the leading bits (exact in the narrow
+// format), lo = x - hi is exact in FP32 and rounded once to the narrow format.
+__device__ __forceinline__ float trunc_bits(float x, uint32_t mask) {
+  return __uint_as_float(__float_as_uint(x) & mask);
+}
+
+// MFMA of one 16-step chunk.  x[8] are this lane's 8 row values (steps 8h + k,
+// or 2k + h for the interleaved FP32 form).
 template <int PREC>
 struct ChunkMfma;
 
 template <>
 struct ChunkMfma<JNE_PREC_MFMA_FP16X2> {
   static constexpr bool kInterleaved = false;
-  static constexpr bool kPackedConst = true;  // rows 1, k, k(k-1)/2 (< 2048) are exact fp16: lo = 0
-  __device__ __forceinline__ static void split(const float (&x)[8], u32x4v& hi, u32x4v& lo) {
-    half8 h, l;
+  __device__ __forceinline__ static void run(const float (&x)[8], floatx16& acc, floatx16& acc2) {
+    half8 hi, lo;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      h[k] = (_Float16)x[k];
-      l[k] = (_Float16)(x[k] - (float)h[k]);
+#if JNE_MFMA_SPLIT_ROUND
+      hi[k] = (_Float16)x[k];
+      lo[k] = (_Float16)(x[k] - (float)hi[k]);
+#else
+      const float h = trunc_bits(x[k], 0xFFFFE000u);  // 11 significant bits
+      hi[k] = (_Float16)h;
+      lo[k] = (_Float16)(x[k] - h);
+#endif
     }
-    hi = __builtin_bit_cast(u32x4v, h);
-    lo = __builtin_bit_cast(u32x4v, l);
-  }
-  __device__ __forceinline__ static void mma(const u32x4v& hi, const u32x4v& lo, floatx16& acc, floatx16& acc2) {
-    const half8 h = __builtin_bit_cast(half8, hi), l = __builtin_bit_cast(half8, lo);
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(h, h, acc, 0, 0, 0);
-    acc2 = __builtin_amdgcn_mfma_f32_32x32x16_f16(h, l, acc2, 0, 0, 0);
-    acc2 = __builtin_amdgcn_mfma_f32_32x32x16_f16(l, h, acc2, 0, 0, 0);
-  }
-  __device__ __forceinline__ static void run(const float (&x)[8], floatx16& acc, floatx16& acc2) {
-    u32x4v hi, lo;
-    split(x, hi, lo);
-    mma(hi, lo, acc, acc2);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(hi, hi, acc, 0, 0, 0);
+    acc2 = __builtin_amdgcn_mfma_f32_32x32x16_f16(hi, lo, acc2, 0, 0, 0);
+    acc2 = __builtin_amdgcn_mfma_f32_32x32x16_f16(lo, hi, acc2, 0, 0, 0);
+#if JNE_MFMA_LOLO
+    acc2 = __builtin_amdgcn_mfma_f32_32x32x16_f16(lo, lo, acc2, 0, 0, 0);
+#endif
   }
 };
 
 template <>
 struct ChunkMfma<JNE_PREC_MFMA_BF16X2> {
   static constexpr bool kInterleaved = false;
-  static constexpr bool kPackedConst = false;
   __device__ __forceinline__ static void run(const float (&x)[8], floatx16& acc, floatx16& acc2) {
     bf16x8 hi, lo;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      hi[k] = (__bf16)x[k];
-      lo[k] = (__bf16)(x[k] - (float)hi[k]);
+      const float h = trunc_bits(x[k], 0xFFFF0000u);  // 8 significant bits
+      hi[k] = (__bf16)h;
+      lo[k] = (__bf16)(x[k] - h);
     }
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi, hi, acc, 0, 0, 0);
     acc2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi, lo, acc2, 0, 0, 0);
@@ -119,15 +136,16 @@ struct ChunkMfma<JNE_PREC_MFMA_BF16X2> {
 template <>
 struct ChunkMfma<JNE_PREC_MFMA_BF16X3> {
   static constexpr bool kInterleaved = false;
-  static constexpr bool kPackedConst = false;
   __device__ __forceinline__ static void run(const float (&x)[8], floatx16& acc, floatx16& acc2) {
     bf16x8 hi, mi, lo;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      hi[k] = (__bf16)x[k];
-      const float r1 = x[k] - (float)hi[k];
-      mi[k] = (__bf16)r1;
-      lo[k] = (__bf16)(r1 - (float)mi[k]);
+      const float h = trunc_bits(x[k], 0xFFFF0000u);
+      const float r1 = x[k] - h;
+      const float m = trunc_bits(r1, 0xFFFF0000u);
+      hi[k] = (__bf16)h;
+      mi[k] = (__bf16)m;
+      lo[k] = (__bf16)(r1 - m);
     }
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi, hi, acc, 0, 0, 0);
     acc2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi, mi, acc2, 0, 0, 0);
@@ -141,11 +159,21 @@ struct ChunkMfma<JNE_PREC_MFMA_BF16X3> {
 template <>
 struct ChunkMfma<JNE_PREC_MFMA_FP32> {
   static constexpr bool kInterleaved = true;
-  static constexpr bool kPackedConst = false;
   __device__ __forceinline__ static void run(const float (&x)[8], floatx16& acc, floatx16&) {
 #pragma unroll
     for (int m = 0; m < 8; ++m) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(x[m], x[m], acc, 0, 0, 0);
   }
+};
+
+// Philox calls of a segment unrolled: by-seed form only (the from-normals
+// form has no Philox chains to overlap), whole iterations only, and not with
+// the three-way bf16 split, whose operands leave no registers for it
+template <int D, int PREC, bool FROM_NORMALS>
+struct CallUnroll {
+  static constexpr int value = (JNE_MFMA_UNROLL_CALLS && !FROM_NORMALS && PREC != JNE_PREC_MFMA_BF16X3 &&
+                                MfmaShape<D>::CALLS % 64 == 0)
+                                   ? MfmaShape<D>::IT
+                                   : 1;
 };
 
 template <bool B>
@@ -163,21 +191,23 @@ __device__ __forceinline__ float partner_half(float v, int h) {
 }
 
 template <int D, int PREC, bool FROM_NORMALS>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void walk_moments_mfma(const uint32_t* __restrict__ seeds,
-                                                        const double* __restrict__ zin, uint32_t rng_stream,
-                                                        uint32_t T, uint32_t n_walks, double* __restrict__ acc,
-                                                        size_t ld) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(JNE_MFMA_WAVES)))
+void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restrict__ zin, uint32_t rng_stream,
+                       uint32_t T, uint32_t n_walks, double* __restrict__ acc, size_t ld) {
   using Sh = MfmaShape<D>;
   constexpr int ZP = Sh::ZP, NG = Sh::NG, EPS_P = Sh::EPS_P;
   using Op = ChunkMfma<PREC>;
 
-  __shared__ __align__(16) unsigned char smem[Sh::SMEM];
-  float(*zbuf)[ZP] = reinterpret_cast<float(*)[ZP]>(smem);  // one segment's normals, component-major
-  double(*pb)[33] = reinterpret_cast<double(*)[33]>(smem);  // the final Gram (after the last segment)
+  // rows 0..D-1: the segment's normals (component-major); rows D..D+3: the
+  // constant rows 1, k, k(k-1)/2 and a zero row, written once
+  __shared__ __align__(16) float zbuf[Sh::ZROWS][ZP];
+  __shared__ double gram[kGramSize];  // FP64 Gram (compact), flushed into every segment
   __shared__ double eps_part[EPS_P][D];
   __shared__ double seg_eps[D];
   __shared__ double seg_sig[D];
-  __shared__ double b0s[D];
+  __shared__ double b0s[D];      // B at the segment start (FP64)
+  __shared__ double seg_b0c[D];  // B0 + s m: the state the centred c rows are relative to
+  __shared__ float seg_m[D];
 
   const int l = threadIdx.x;
   const int row = l & 31;  // also the accumulator column
@@ -190,36 +220,34 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void wa
 
   const bool is_z = row < D;
   const bool is_c = row >= kRowC && row < kRowC + D;
-  const bool use_const = !is_z && !is_c;  // rows 1, k, k(k-1)/2 and zero rows
-  const int comp = is_z ? row : is_c ? row - kRowC : 0;
-  const float w1 = row == kRowDet ? 1.0f : 0.0f;
-  const float wk = row == kRowDet + 1 ? 1.0f : 0.0f;
-  const float wkk = row == kRowDet + 2 ? 1.0f : 0.0f;
+  // LDS row feeding this lane's row of X (the c rows scan the z rows)
+  const int zr = is_z ? row : is_c ? row - kRowC : (row >= kRowDet && row < kRowDet + 3) ? D + row - kRowDet : D + 3;
   const float hf = (float)h;
+  // c rows take the scan, the others the LDS row.  A bitwise blend: written as
+  // a select, the compiler turns it into a lane-indexed stack array
+  const uint32_t cmask = is_c ? 0xFFFFFFFFu : 0u;
 
-  // step index (within the segment) of value k of chunk ch on this lane
-  auto kidx = [&](int ch, int k) -> int { return 16 * ch + (Op::kInterleaved ? 2 * k + h : 8 * h + k); };
-  // the value of row 1 / k / k(k-1)/2 (0 on other rows) at segment step kk
-  auto det_value = [&](int kk) -> float {
-    const float kf = (float)kk;
-    return __builtin_fmaf(wkk, kf * (kf - 1.0f) * 0.5f, __builtin_fmaf(wk, kf, w1));
-  };
-
-  // fp16: the constant rows' hi operands of the 4 chunks, packed, in LDS (their
-  // lo part is 0); slot (row - 24) * 2 + h for rows 24..26, slot 6 all zero
-  __shared__ u32x4v cst[kChunks][7];
-  const int cslot = row >= kRowDet && row < kRowDet + 3 ? (row - kRowDet) * 2 + h : 6;
-  if constexpr (Op::kPackedConst) {
-#pragma unroll
-    for (int ch = 0; ch < kChunks; ++ch) {
-      float xv[8];
-#pragma unroll
-      for (int k = 0; k < 8; ++k) xv[k] = det_value(kidx(ch, k));
-      u32x4v hi, lo;
-      Op::split(xv, hi, lo);
-      cst[ch][cslot] = hi;  // lanes sharing slot 6 all write zeros
-    }
+  // constant rows, exact small integers (k(k-1)/2 <= 2016)
+  for (int t = l; t < ZP; t += 64) {
+    const float kf = (float)t;
+    zbuf[D][t] = 1.0f;
+    zbuf[D + 1][t] = kf;
+    zbuf[D + 2][t] = kf * (kf - 1.0f) * 0.5f;
+    zbuf[D + 3][t] = 0.0f;
   }
+  for (int e = l; e < kGramSize; e += 64) gram[e] = 0.0;
+
+  // accumulator v of lane l is Gram element [8(v/4) + 4h + v%4][l%32]: this
+  // lane's compact base and the entries it keeps
+  auto needed = [&](int r) -> bool {
+    const bool rz = r < D, rc = r >= kRowC && r < kRowC + D, rd = r >= kRowDet && r < kRowDet + 3;
+    return (is_c && (rz || rc || rd)) || (is_z && rd);
+  };
+  const int gbase = (is_c ? (row - kRowC) * 27 : kGramZ + 3 * row - kRowDet) + 4 * h;
+  uint32_t gmask = 0;
+#pragma unroll
+  for (int v = 0; v < 16; ++v)
+    if (needed(8 * (v / 4) + 4 * h + (v % 4))) gmask |= 1u << v;
 
   // (i, j) grid entries of this lane: e = l + 64 g
   int gi[NG], gj[NG];
@@ -240,11 +268,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void wa
   const double s = sqrt(dt);
   const uint32_t rounds = (T + kSegSteps - 1) / kSegSteps;
 
-  floatx16 accm = {}, accc = {};
-  double g64[16];  // FP64 Gram, flushed from the MFMA accumulators once per segment
-#pragma unroll
-  for (int v = 0; v < 16; ++v) g64[v] = 0.0;
-
   // one segment; the partial last segment (T not a multiple of 64) runs a copy
   // with the step masks, so the loop over full segments carries none of them
   auto segment = [&](uint32_t r, auto tail_tag) {
@@ -252,11 +275,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void wa
     const uint32_t ts = r * kSegSteps;
     // ---- normals of the segment -> LDS
     CallPos<D> pos = pos0;
-#if JNE_MFMA_UNROLL_CALLS
-#pragma unroll
-#else
-#pragma unroll 1
-#endif
+#pragma unroll CallUnroll<D, PREC, FROM_NORMALS>::value
     for (int j = 0; j < Sh::IT; ++j) {
       const int c = l + 64 * j;
       if (Sh::IT * 64 != Sh::CALLS && c >= Sh::CALLS) break;
@@ -278,9 +297,32 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void wa
     }
     __syncthreads();
 
+    // ---- eps = sum of the segment's normals in FP64; lane (i, p) sums steps
+    // p, p+P, ... of component i.  The c rows are centred on m = eps/2 (an FP32
+    // value): c'_t = c_t - m halves their mean square, so the FP32 accumulation
+    // rounds less, and b_t = (B0 + s m) + s c'_t
+    if (l < EPS_P * D) {
+      const int i = l % D, p = l / D;
+      double e = 0.0;
+      for (int t = p; t < kSegSteps; t += EPS_P) e += (double)zbuf[i][t];
+      eps_part[p][i] = e;
+    }
+    __syncthreads();
+    if (l < D) {
+      double e = 0.0;
+#pragma unroll
+      for (int p = 0; p < EPS_P; ++p) e += eps_part[p][l];
+      seg_eps[l] = e;
+      const float m = (float)(0.5 * e);
+      seg_m[l] = m;
+      seg_b0c[l] = b0s[l] + s * (double)m;
+    }
+    __syncthreads();
+
     // ---- 4 chunks of 16 steps on the matrix cores
-    float carry = 0.0f;
-    const float* zrow = zbuf[comp];
+    floatx16 accm = {}, accc = {};
+    float carry = is_c ? -seg_m[row - kRowC] : 0.0f;
+    const float* zrow = zbuf[zr];
 #pragma unroll 1  // unrolled, the four chunks' loads and operands exceed the register budget
     for (int ch = 0; ch < kChunks; ++ch) {
       const int off = 16 * ch;
@@ -315,80 +357,47 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void wa
       }
       float x[8];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) x[k] = is_c ? cv[k] : zm[k];
-      if constexpr (!tail) {
-        if constexpr (Op::kPackedConst) {
-          u32x4v hi, lo;
-          Op::split(x, hi, lo);
-          const u32x4v cv4 = cst[ch][cslot];
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            hi[q] = use_const ? cv4[q] : hi[q];
-            lo[q] = use_const ? 0u : lo[q];
-          }
-          Op::mma(hi, lo, accm, accc);
-        } else {
-#pragma unroll
-          for (int k = 0; k < 8; ++k) x[k] = use_const ? det_value(kidx(ch, k)) : x[k];
-          Op::run(x, accm, accc);
+      for (int k = 0; k < 8; ++k) {
+        x[k] = __uint_as_float((__float_as_uint(cv[k]) & cmask) | (__float_as_uint(zm[k]) & ~cmask));
+        if constexpr (tail) {  // steps at or beyond T contribute nothing
+          const int kk = off + (Op::kInterleaved ? 2 * k + h : 8 * h + k);
+          x[k] = ts + (uint32_t)kk < T ? x[k] : 0.0f;
         }
-      } else {  // steps at or beyond T contribute nothing
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-          const int kk = kidx(ch, k);
-          const float xv = use_const ? det_value(kk) : x[k];
-          x[k] = ts + (uint32_t)kk < T ? xv : 0.0f;
-        }
-        Op::run(x, accm, accc);
       }
+      Op::run(x, accm, accc);
     }
 
     // ---- segment end
-    // eps = sum of the segment's normals in FP64; lane (i, p) sums steps p, p+P, ... of component i
-    if (l < EPS_P * D) {
-      const int i = l % D, p = l / D;
-      double e = 0.0;
-      for (int t = p; t < kSegSteps; t += EPS_P) e += (double)zbuf[i][t];
-      eps_part[p][i] = e;
-    }
     float f[16];
 #pragma unroll
-    for (int v = 0; v < 16; ++v) {
-      f[v] = accm[v] + accc[v];
-      accm[v] = 0.0f;
-      accc[v] = 0.0f;
-    }
-    // accumulator v of lane l is Gram element [8(v/4) + 4h + v%4][l%32]; v = 12 on
-    // the lower half is row 24 (the ones row): f[12] there is sum_seg x_col
-    if (h == 0 && is_c) seg_sig[comp] = (double)f[12];
-    __syncthreads();
-    if (l < D) {
-      double e = 0.0;
-#pragma unroll
-      for (int p = 0; p < EPS_P; ++p) e += eps_part[p][l];
-      seg_eps[l] = e;
-    }
+    for (int v = 0; v < 16; ++v) f[v] = accm[v] + accc[v];
+    // f[12] on the lower half is row 24 (the ones row): sum_seg x_col
+    if (h == 0 && is_c) seg_sig[row - kRowC] = (double)f[12];
     __syncthreads();
     const uint32_t te = tail ? T : ts + kSegSteps;
     const double nseg = (double)(te - ts);
     {
-      // rows 24..26 (v = 12..14, lower half): sum x, sum k x, sum k(k-1)/2 x ->
-      // sum x, sum u x, sum u^2 x with u = (a + k)/T, a = ts + 1
+      // flush into the FP64 Gram.  Rows 24..26 (v = 12..14, lower half) hold
+      // sum x, sum k x, sum k(k-1)/2 x: stored as sum x, sum u x, sum u^2 x with
+      // u = (a + k)/T, a = ts + 1; on the z columns sum x is the FP64 eps.
+      const double a = (double)ts + 1.0;
+      const double P0 = is_z ? seg_eps[row] : (double)f[12];
+      const double P1 = (double)f[13], P2 = (double)f[14];
+      double add[16];
+#pragma unroll
+      for (int v = 0; v < 16; ++v) add[v] = (double)f[v];
+      add[12] = P0;
+      add[13] = (a * P0 + P1) * dt;
+      add[14] = ((a * a) * P0 + (2.0 * a + 1.0) * P1 + 2.0 * P2) * (dt * dt);
 #pragma unroll
       for (int v = 0; v < 16; ++v)
-        if (v < 12 || v > 14) g64[v] += (double)f[v];
-      const double a = (double)ts + 1.0;
-      const double P0 = (h == 0 && is_z) ? seg_eps[comp] : (double)f[12];
-      const double P1 = (double)f[13], P2 = (double)f[14];
-      g64[12] += P0;
-      g64[13] += (a * P0 + P1) * dt;
-      g64[14] += ((a * a) * P0 + (2.0 * a + 1.0) * P1 + 2.0 * P2) * (dt * dt);
+        if (gmask & (1u << v)) gram[gbase + 8 * (v / 4) + (v % 4)] += add[v];
     }
     // FP64 per-segment terms on the (i, j) grid
 #pragma unroll
     for (int g = 0; g < NG; ++g) {
       if (l + 64 * g < D * D) {
-        const double bi = b0s[gi[g]], bj = b0s[gj[g]];
+        const double bi = seg_b0c[gi[g]], bj = seg_b0c[gj[g]];
         const double si = seg_sig[gi[g]], sj = seg_sig[gj[g]];
         bbA[g] += nseg * (bi * bj) + s * ((bi * sj) + (si * bj));
         ebA[g] += (s * seg_eps[gi[g]]) * bj;
@@ -401,20 +410,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void wa
                         : mp == 1 ? (bnd * (bnd + 1.0) - a * (a + 1.0)) * 0.5 * dt
                                   : (bnd * (bnd + 1.0) * (2.0 * bnd + 1.0) - a * (a + 1.0) * (2.0 * a + 1.0)) /
                                         6.0 * dt * dt;
-      mbA += b0s[mi] * up;
+      mbA += seg_b0c[mi] * up;
     }
     __syncthreads();
+    // the next reads of b0s come two barriers into the next segment
     if (l < D) b0s[l] += s * seg_eps[l];
-    __syncthreads();
   };
   const uint32_t full = T / kSegSteps;
   for (uint32_t r = 0; r < full; ++r) segment(r, BoolTag<false>{});
   if (full < rounds) segment(full, BoolTag<true>{});
 
-  // the Gram to LDS (over the normals buffer)
-#pragma unroll
-  for (int v = 0; v < 16; ++v) pb[8 * (v / 4) + 4 * h + (v % 4)][row] = g64[v];
-  __syncthreads();
   const double s2 = s * s;
   // canonical layout: [0, D^2) sum b b^T, [D^2, 2D^2) sum e b^T, then
   // sum b, sum u b, sum u^2 b, sum e, sum u e, sum u^2 e
@@ -423,14 +428,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void wa
     const int e = l + 64 * g;
     if (e < D * D) {
       const int i = gi[g], j = gj[g];
-      const int lo = i < j ? i : j, hi = i < j ? j : i;  // read one triangle: exactly symmetric
-      acc[(size_t)e * ld + walk] = bbA[g] + s2 * pb[kRowC + lo][kRowC + hi];
-      acc[(size_t)(D * D + e) * ld + walk] = ebA[g] + s2 * pb[i][kRowC + j];
+      const int lo = i < j ? i : j, hi = i < j ? j : i;  // one triangle: exactly symmetric
+      acc[(size_t)e * ld + walk] = bbA[g] + s2 * gram[hi * 27 + kRowC + lo];
+      acc[(size_t)(D * D + e) * ld + walk] = ebA[g] + s2 * gram[j * 27 + i];
     }
   }
   if (l < 3 * D) {
-    acc[(size_t)(2 * D * D + mp * D + mi) * ld + walk] = mbA + s * pb[kRowDet + mp][kRowC + mi];
-    acc[(size_t)(2 * D * D + (3 + mp) * D + mi) * ld + walk] = s * pb[kRowDet + mp][mi];
+    acc[(size_t)(2 * D * D + mp * D + mi) * ld + walk] = mbA + s * gram[mi * 27 + kRowDet + mp];
+    acc[(size_t)(2 * D * D + (3 + mp) * D + mi) * ld + walk] = s * gram[kGramZ + 3 * mi + mp];
   }
 }
 

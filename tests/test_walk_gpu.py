@@ -1,9 +1,11 @@
 """GPU parity of the FP64 walk path against the oracle, through the C ABI.
 
-Tolerances: eigenvalues within 1e-9 relative of the oracle (reference
-algorithm + LAPACK dggev on the same increments); the structural zero
-eigenvalue of models 1 and 3 is compared with an absolute floor of
-1e-9 * rho_max.  Normals are compared bit for bit.
+Tolerances: |rho_gpu - rho_oracle| <= 1e-9 |rho| + 1e-13 rho_max per entry
+against the oracle (reference algorithm + LAPACK dggev on the same
+increments).  The absolute term covers the structural zero eigenvalue of
+models 1 and 3 and eigenvalues many orders below rho_max, which two
+backward-stable solvers agree on only to about eps * rho_max.  Normals are
+compared bit for bit.
 """
 import numpy as np
 import pytest
