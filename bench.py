@@ -7,8 +7,13 @@ steps, 1,000,000 walks per step (seeds 1..1,000,000, Philox stream 0).  One
 one-pass moments -> model algebra -> FP64 eigen solve, with the seeds already
 resident in HBM and the eigenvalues left in HBM.
 
+The headline path is the mixed-precision one (moment Gram on the matrix cores,
+fp16 hi/lo operands, FP32 accumulate flushed to FP64 every 64 steps; FP64
+state, model algebra and eigen solve).  The plain FP64 path is timed on the
+same batch in the same run, and the two outputs are compared walk by walk.
+
 Prints one JSON line with walks/s, the roofline figure of the dominant kernel
-(timed with HIP events on its own stream) and the CPU baseline (the C
+(timed with HIP events on the stream it runs on) and the CPU baseline (the C
 restatement of the reference walk in oracle/, on a bounded sample of the same
 workload, on this host's cores).
 """
@@ -25,7 +30,14 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 METRIC = "eigen-solved walks/s @dim12 T=10k model4 vs FP64 GPU; p50/p99 query latency"
-FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
+# MI355X published dense peaks, TFLOP/s
+PEAK_TFLOPS = {"fp64": 78.6, "mfma_fp16x2": 2500.0, "mfma_bf16x2": 2500.0, "mfma_bf16x3": 2500.0,
+               "mfma_fp32": 157.3}
+DTYPE = {"fp64": "f64", "mfma_fp16x2": "fp16 hi/lo MFMA (fp32 accumulate) + f64",
+         "mfma_bf16x2": "bf16 hi/lo MFMA (fp32 accumulate) + f64",
+         "mfma_bf16x3": "bf16 hi/mid/lo MFMA (fp32 accumulate) + f64", "mfma_fp32": "fp32 MFMA + f64"}
+# MFMA operations issued per 16-step chunk of one walk (32x32x16 products)
+MFMA_PER_CHUNK = {"mfma_fp16x2": 3, "mfma_bf16x2": 3, "mfma_bf16x3": 6}
 
 
 def algorithmic_flops_per_walk(model: int, dim: int, steps: int) -> int:
@@ -33,6 +45,15 @@ def algorithmic_flops_per_walk(model: int, dim: int, steps: int) -> int:
     them (src/johansen_statistics.rs:32-33)."""
     nf = dim + 1 if model in (1, 3) else dim
     return 2 * steps * nf * (nf + dim)
+
+
+def relative_floor(got, ref, floor=1e-5):
+    """|d_i| / |rho_i|, with rho_max for entries below floor * rho_max."""
+    import numpy as np
+
+    rmax = np.max(np.abs(ref), axis=-1, keepdims=True)
+    den = np.where(np.abs(ref) >= floor * rmax, np.abs(ref), rmax)
+    return np.abs(got - ref) / den
 
 
 def cpu_baseline(model, dim, steps, threads, target_s=12.0):
@@ -53,6 +74,30 @@ def cpu_baseline(model, dim, steps, threads, target_s=12.0):
     return n / dt, n, dt, eig
 
 
+def time_path(pkg, eng, prec_name, model, dim, T, seeds, out, n, steps, warmup, stream, torch):
+    """Runs warmup + steps passes; returns (walks/s, s/step, moments kernel (ms, launches, walks), eigen ms)."""
+    prec = pkg.PRECISIONS[prec_name]
+
+    def step():
+        eng.eigenvalues_by_seed_device(model, dim, T, seeds.data_ptr(), n, out.data_ptr(), precision=prec,
+                                       hip_stream=stream)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    eng.profile_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    mom = eng.profile_read(pkg.KERNEL_MOMENTS)
+    eig = eng.profile_read(pkg.KERNEL_EIGEN)
+    eng.profile_enable(False)
+    return steps * n / (t1 - t0), (t1 - t0) / steps, mom, eig[0]
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -62,9 +107,13 @@ def main() -> int:
     ap.add_argument("--model", type=int, default=4)
     ap.add_argument("--dim", type=int, default=12)
     ap.add_argument("--T", type=int, default=10_000, help="time steps per walk")
-    ap.add_argument("--precision", default="fp64")
+    ap.add_argument("--precision", default="mfma_fp16x2", help="headline path")
+    ap.add_argument("--compare", default="fp64", help="path timed beside it ('' to skip)")
+    ap.add_argument("--check-walks", type=int, default=10_000, help="walks compared between the two paths")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cores)")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="also run one device copy of this many MiB (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -80,7 +129,6 @@ def main() -> int:
     import __graft_entry__
 
     pkg = __graft_entry__.import_package()
-    prec = pkg.PRECISIONS[args.precision]
     model, dim, T, n = args.model, args.dim, args.T, args.walks
     nf = pkg.num_eigenvalues(model, dim)
 
@@ -91,53 +139,70 @@ def main() -> int:
     out = torch.empty(n * nf, dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream(dev).cuda_stream
 
-    def step():
-        eng.eigenvalues_by_seed_device(model, dim, T, seeds.data_ptr(), n, out.data_ptr(), precision=prec,
-                                       hip_stream=stream)
+    if args.copy_probe_mb:
+        src = torch.empty(args.copy_probe_mb << 20, dtype=torch.uint8, device=dev).fill_(1)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    eng.profile_enable(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    mom_ms, mom_launches, mom_walks = eng.profile_read(pkg.KERNEL_MOMENTS)
-    eig_ms, eig_launches, eig_walks = eng.profile_read(pkg.KERNEL_EIGEN)
-    eng.profile_enable(False)
+    wps, sps, mom, eig_ms = time_path(pkg, eng, args.precision, model, dim, T, seeds, out, n, args.steps,
+                                      args.warmup, stream, torch)
+    k = min(args.check_walks, n)
+    head = out.view(n, nf)[:k].cpu().numpy()
+    assert np.all(np.isfinite(head)), "non-finite eigenvalues"
 
-    elapsed = t1 - t0
-    walks_per_s = args.steps * n / elapsed
     fpw = algorithmic_flops_per_walk(model, dim, T)
+    mom_ms, mom_launches, mom_walks = mom
     avg_ms = mom_ms / max(mom_launches, 1)
     walks_per_launch = mom_walks / max(mom_launches, 1)
     achieved = fpw * walks_per_launch / (avg_ms * 1e-3) / 1e12
+    peak = PEAK_TFLOPS[args.precision]
+    kernel = (f"walk_moments_fp64<{dim}>" if args.precision == "fp64"
+              else f"walk_moments_mfma<{dim}, {args.precision}>")
 
-    gpu_eigs = out.view(n, nf)[:64].cpu().numpy()
-    assert np.all(np.isfinite(gpu_eigs)), "non-finite eigenvalues"
-
+    # HBM bytes per launch from the committed counter runs (per-walk figures,
+    # copy-calibrated) times this run's walks per launch
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic.json")
     key = f"walk_moments_{args.precision}_m{model}_d{dim}_T{T}"
     if os.path.exists(tfile):
         with open(tfile) as f:
-            traffic = json.load(f).get(key)
+            rec = json.load(f).get(key)
+        if rec:
+            traffic = (rec["fetch_bytes_per_walk"] + rec["write_bytes_per_walk"]) * walks_per_launch
+
+    roofline = {
+        "bound": "mfma",
+        "achieved": achieved,
+        "peak": peak,
+        "unit": "TFLOP/s",
+        "frac": achieved / peak,
+        "traffic": traffic,
+        "traffic_unit": "bytes per launch (HBM, rocprofv3 FETCH_SIZE + WRITE_SIZE)",
+        "algorithmic_bytes_per_launch": (4 + 8 * (2 * dim * dim + 6 * dim)) * walks_per_launch,
+        "kernel": kernel,
+        "algorithmic_flop_per_walk": fpw,
+        "avg_launch_ms": avg_ms,
+        "walks_per_launch": walks_per_launch,
+        "normals_per_s": wps * T * dim,
+    }
+    if args.precision in MFMA_PER_CHUNK:
+        issued = MFMA_PER_CHUNK[args.precision] * 2 * 32 * 32 * 16 * ((T + 15) // 16)
+        roofline["issued_mfma_tflops"] = issued * walks_per_launch / (avg_ms * 1e-3) / 1e12
 
     line = {
         "metric": METRIC,
-        "value": walks_per_s,
+        "value": wps,
         "unit": "walks/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": sps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else args.precision,
+        "dtype": DTYPE[args.precision],
         "data": "synthetic (device Philox4x32-10 normals, seeds 1..N)",
         "config": {
             "workload": f"Johansen model {model}, dim {dim}, T={T:,} steps, {n:,} walks per step, "
@@ -148,27 +213,32 @@ def main() -> int:
             "walks_per_step": n,
             "precision": args.precision,
         },
-        "roofline": {
-            "bound": "mfma",
-            "achieved": achieved,
-            "peak": FP64_PEAK_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": traffic,
-            "kernel": f"walk_moments_fp64<{dim}>",
-            "algorithmic_flop_per_walk": fpw,
-            "avg_launch_ms": avg_ms,
-            "walks_per_launch": walks_per_launch,
-        },
-        "kernels_ms": {"moments": mom_ms, "eigen": eig_ms, "launches": mom_launches + eig_launches},
+        "roofline": roofline,
+        "kernels_ms": {"moments": mom_ms, "eigen": eig_ms, "launches": mom_launches},
     }
+
+    if args.compare:
+        cwps, csps, cmom, ceig_ms = time_path(pkg, eng, args.compare, model, dim, T, seeds, out, n, args.steps,
+                                              args.warmup, stream, torch)
+        ref = out.view(n, nf)[:k].cpu().numpy()
+        rel = relative_floor(head, ref)
+        line["compare"] = {
+            "precision": args.compare,
+            "value": cwps,
+            "ms_per_step": csps * 1e3,
+            "moments_avg_launch_ms": cmom[0] / max(cmom[1], 1),
+            "speedup": wps / cwps,
+            "max_rel_diff": float(rel.max()),
+            "mean_rel_diff": float(rel.mean()),
+            "walks_compared": k,
+        }
+        fp64_head = ref if args.compare == "fp64" else None
+    else:
+        fp64_head = head if args.precision == "fp64" else None
 
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         rate, cn, cdt, ceig = cpu_baseline(model, dim, T, threads)
-        # the GPU walks of seeds 1..64 against the CPU restatement
-        k = min(64, cn)
-        rel = np.max(np.abs(gpu_eigs[:k] - ceig[:k]) / np.maximum(np.abs(ceig[:k]), 1e-9 * ceig[:k].max()))
         line["cpu_baseline"] = {
             "value": rate,
             "unit": "walks/s",
@@ -176,8 +246,13 @@ def main() -> int:
             "kind": "port",
             "sample": f"{cn:,} walks (seeds 1..{cn:,}) of the same workload in {cdt:.1f} s; C restatement of "
                       f"the reference CPU walk (oracle/jne_oracle.c), one walk per thread",
-            "gpu_vs_cpu_max_rel_diff_first_64": float(rel),
         }
+        m = min(64, cn, k)
+        if fp64_head is not None:
+            d = np.abs(fp64_head[:m] - ceig[:m])
+            line["cpu_baseline"]["fp64_gpu_vs_cpu_max_rel_diff_first_64"] = float(
+                (d / (np.abs(ceig[:m]) + 1e-13 * ceig[:m].max(axis=1, keepdims=True))).max())
+        line["cpu_baseline"]["headline_vs_cpu_max_rel_diff_first_64"] = float(relative_floor(head[:m], ceig[:m]).max())
     print(json.dumps(line), flush=True)
     eng.close()
     return 0
