@@ -118,6 +118,95 @@ int jne_synchronize(jne_ctx *ctx);
 int jne_profile_enable(jne_ctx *ctx, int enable);
 int jne_profile_read(jne_ctx *ctx, int kernel, double *total_ms, uint64_t *launches, uint64_t *walks);
 
+/* ------------------------------------------------------------------------
+ * Storage: the append-only .dat V6 file (DATA_FORMAT.md), its reader, resume
+ * and percentiles.  Host-only calls; they do not need a context.  On failure
+ * they return a negative code and jne_host_last_error() (per thread) holds
+ * the message; the messages of the reference's io::Error values are kept.
+ * ------------------------------------------------------------------------ */
+#define JNE_ERR_FORMAT (-7) /* file content: magic, parameters or eigenvalue counts */
+
+/* Message of the last failed host call on this thread. */
+const char *jne_host_last_error(void);
+
+typedef struct jne_dat_writer jne_dat_writer;
+
+/* Open `path` for appending records of (model, dim, steps).
+ * Replaces AppendOnlyWriter::with_expected_size (src/data_storage/writer.rs:27-168)
+ * and remove_eof_marker (writer.rs:170-195):
+ *   - a new file gets the 18-byte header;
+ *   - an existing V6 file with the same model, dim and steps is resumed: its
+ *     whole records are kept and the EOF trailer is removed.  Unlike the
+ *     reference, an incomplete trailing record (a crash mid-write) is
+ *     truncated away too, so appended records stay readable;
+ *   - a file whose magic differs is deleted and recreated;
+ *   - different parameters: JNE_ERR_FORMAT, "Model mismatch: file has model
+ *     1, expected 2" (and the Dimension / Steps variants).
+ * expected_records > 0 only prints the reference's pre-allocation notice.
+ * quiet = 0 prints the reference's messages on stdout. */
+int jne_dat_writer_open(const char *path, int model, int dim, uint32_t steps, uint64_t expected_records, int quiet,
+                        jne_dat_writer **out);
+/* Records in the file (found when opened plus appended). */
+uint64_t jne_dat_writer_count(const jne_dat_writer *w);
+/* Print the reference's "Simulation progress: ..." line each time the record
+ * count crosses a multiple of 10,000 (src/data_storage/thread_manager.rs:59-83),
+ * against total_runs; 0 turns it off. */
+int jne_dat_writer_set_progress(jne_dat_writer *w, uint64_t total_runs);
+/* Append n records: seeds[i] with eigenvalues[i*nf .. i*nf+nf).
+ * Replaces append_eigenvalues (writer.rs:198-253); nf must stay the same for
+ * the whole file ("Eigenvalue count mismatch: ...") and be <= 255. */
+int jne_dat_writer_append(jne_dat_writer *w, const uint32_t *seeds, const double *eigenvalues, size_t n, int nf);
+/* Write the trailer (EOF_MARK, record count, nF), close and free w.
+ * Replaces AppendOnlyWriter::finish (writer.rs:256-302). */
+int jne_dat_writer_finish(jne_dat_writer *w);
+/* Flush, close and free w without a trailer (an interrupted run). */
+void jne_dat_writer_abandon(jne_dat_writer *w);
+
+/* Header and record count of a .dat file (trailer fast path, or a scan of
+ * whole records when the trailer is missing).  Replaces the header/metadata
+ * part of read_append_file (src/data_storage/reader.rs:23-104). */
+int jne_dat_read_info(const char *path, int *model, int *dim, uint32_t *steps, uint64_t *records, int *nf,
+                      int *has_trailer);
+/* All records in file order: seeds[n_read], eigenvalues[n_read * nf] (either
+ * may be NULL to skip it).  Replaces read_append_file (reader.rs:23-216);
+ * capacity is the number of records the arrays hold. */
+int jne_dat_read(const char *path, uint32_t *seeds, double *eigenvalues, uint64_t capacity, uint64_t *n_read,
+                 int *nf);
+/* Seeds 1..=runs that `path` does not hold yet, ascending, into out[capacity]
+ * (out may be NULL to count them); *completed = records in the file.
+ * Replaces check_append_progress + get_remaining_seeds
+ * (src/data_storage/progress.rs:10-61): a missing or unreadable file counts as
+ * empty, different parameters are JNE_ERR_FORMAT. */
+int jne_dat_remaining_seeds(const char *path, int model, int dim, uint32_t steps, uint64_t runs, uint32_t *out,
+                            uint64_t capacity, uint64_t *n_out, uint64_t *completed);
+
+/* Percentiles p[0..np) of values[0..n): rank p*(n-1), linear interpolation
+ * between the neighbouring order statistics.  Replaces get_percentile_value
+ * (src/simulation_analyzers.rs:4-18); NaN values are JNE_ERR_NUMERIC. */
+int jne_percentiles(const double *values, size_t n, const double *p, size_t np, double *out);
+
+/* Per-record statistic of a .dat file: JNE_STAT_TRACE = sum of eigenvalues,
+ * JNE_STAT_MAXEIG = largest (src/simulation_analyzers.rs:21-40), for the
+ * records with seed <= runs (runs = 0: all).  out may be NULL to count. */
+#define JNE_STAT_TRACE 0
+#define JNE_STAT_MAXEIG 1
+int jne_dat_statistics(const char *path, uint64_t runs, int statistic, double *out, uint64_t capacity,
+                       uint64_t *n_out);
+
+/* ------------------------------------------------------------------------
+ * Simulation driver: one (model, dim, steps, runs) job streamed to a .dat
+ * file with resume.  Replaces run_model_simulation
+ * (src/data_storage/parallel_compute.rs:150-232) with its writer thread
+ * (src/data_storage/thread_manager.rs:25-92): the remaining seeds are
+ * computed on ctx's device in batches while the previous batch is copied to
+ * pinned host memory and encoded/written on a host thread.  quiet = 0 prints
+ * the reference's progress and status messages.  A file with different
+ * parameters is replaced, as the reference does.  Records are written in
+ * ascending seed order within each batch.
+ * ------------------------------------------------------------------------ */
+int jne_run_simulation(jne_ctx *ctx, int model, int dim, uint32_t steps, uint64_t runs, int precision,
+                       const char *path, int quiet);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,6 +31,10 @@ JNE_ERR_UNSUPPORTED = -3
 JNE_ERR_NOMEM = -4
 JNE_ERR_NUMERIC = -5
 JNE_ERR_IO = -6
+JNE_ERR_FORMAT = -7
+
+STAT_TRACE = 0
+STAT_MAXEIG = 1
 
 PREC_FP64 = 0
 PREC_FP32_KAHAN = 1
@@ -101,8 +105,54 @@ def load_library() -> ctypes.CDLL:
         L.jne_profile_read.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]
         L.jne_profile_read.restype = ctypes.c_int
+        # storage, percentiles and the simulation driver
+        L.jne_host_last_error.argtypes = []
+        L.jne_host_last_error.restype = ctypes.c_char_p
+        L.jne_dat_writer_open.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                          ctypes.c_uint64, ctypes.c_int, c_void_pp]
+        L.jne_dat_writer_open.restype = ctypes.c_int
+        L.jne_dat_writer_count.argtypes = [ctypes.c_void_p]
+        L.jne_dat_writer_count.restype = ctypes.c_uint64
+        L.jne_dat_writer_set_progress.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
+        L.jne_dat_writer_set_progress.restype = ctypes.c_int
+        L.jne_dat_writer_append.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                            ctypes.c_int]
+        L.jne_dat_writer_append.restype = ctypes.c_int
+        L.jne_dat_writer_finish.argtypes = [ctypes.c_void_p]
+        L.jne_dat_writer_finish.restype = ctypes.c_int
+        L.jne_dat_writer_abandon.argtypes = [ctypes.c_void_p]
+        L.jne_dat_writer_abandon.restype = None
+        c_int_p = ctypes.POINTER(ctypes.c_int)
+        c_u32_p = ctypes.POINTER(ctypes.c_uint32)
+        c_u64_p = ctypes.POINTER(ctypes.c_uint64)
+        L.jne_dat_read_info.argtypes = [ctypes.c_char_p, c_int_p, c_int_p, c_u32_p, c_u64_p, c_int_p, c_int_p]
+        L.jne_dat_read_info.restype = ctypes.c_int
+        L.jne_dat_read.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, c_u64_p,
+                                   c_int_p]
+        L.jne_dat_read.restype = ctypes.c_int
+        L.jne_dat_remaining_seeds.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                              ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64, c_u64_p, c_u64_p]
+        L.jne_dat_remaining_seeds.restype = ctypes.c_int
+        L.jne_percentiles.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t,
+                                      ctypes.c_void_p]
+        L.jne_percentiles.restype = ctypes.c_int
+        L.jne_dat_statistics.argtypes = [ctypes.c_char_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p,
+                                         ctypes.c_uint64, c_u64_p]
+        L.jne_dat_statistics.restype = ctypes.c_int
+        L.jne_run_simulation.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                         ctypes.c_uint64, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+        L.jne_run_simulation.restype = ctypes.c_int
         _lib = L
         return L
+
+
+def _host_check(rc: int):
+    if rc != JNE_OK:
+        raise JneError(rc, load_library().jne_host_last_error().decode())
+
+
+def _path(p) -> bytes:
+    return os.fsencode(p)
 
 
 def num_eigenvalues(model: int, dim: int) -> int:
@@ -249,6 +299,161 @@ class Engine:
         self._check(self._lib.jne_profile_read(self._ctx, int(kernel), ctypes.byref(ms), ctypes.byref(nl),
                                                ctypes.byref(nw)))
         return ms.value, nl.value, nw.value
+
+
+class DatWriter:
+    """Append-only .dat V6 writer with resume (src/data_storage/writer.rs:15-302)."""
+
+    def __init__(self, path, model: int, dim: int, steps: int, expected_records: int = 0, quiet: bool = True):
+        self._lib = load_library()
+        w = ctypes.c_void_p()
+        _host_check(self._lib.jne_dat_writer_open(_path(path), int(model), int(dim), int(steps),
+                                                  int(expected_records), 1 if quiet else 0, ctypes.byref(w)))
+        self._w = w
+
+    @property
+    def count(self) -> int:
+        return int(self._lib.jne_dat_writer_count(self._w))
+
+    def set_progress(self, total_runs: int):
+        _host_check(self._lib.jne_dat_writer_set_progress(self._w, int(total_runs)))
+
+    def append(self, seeds, eigenvalues):
+        s = np.ascontiguousarray(seeds, dtype=np.uint32).reshape(-1)
+        e = np.ascontiguousarray(eigenvalues, dtype=np.float64)
+        if e.ndim == 1:
+            e = e.reshape(len(s), -1) if len(s) else e.reshape(0, 0)
+        _host_check(self._lib.jne_dat_writer_append(self._w, s.ctypes.data, e.ctypes.data, len(s),
+                                                    e.shape[1] if e.ndim == 2 else 0))
+
+    def finish(self):
+        w, self._w = self._w, None
+        _host_check(self._lib.jne_dat_writer_finish(w))
+
+    def abandon(self):
+        w, self._w = self._w, None
+        if w:
+            self._lib.jne_dat_writer_abandon(w)
+
+
+def dat_info(path) -> dict:
+    L = load_library()
+    m, d, nf, tr = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    st, n = ctypes.c_uint32(), ctypes.c_uint64()
+    _host_check(L.jne_dat_read_info(_path(path), ctypes.byref(m), ctypes.byref(d), ctypes.byref(st), ctypes.byref(n),
+                                    ctypes.byref(nf), ctypes.byref(tr)))
+    return {"model": m.value, "dim": d.value, "steps": st.value, "records": n.value, "nf": nf.value,
+            "has_trailer": bool(tr.value)}
+
+
+def read_dat(path):
+    """(seeds uint32[n], eigenvalues float64[n, nF], model, dim, steps): read_append_file (reader.rs:23-216)."""
+    info = dat_info(path)
+    n, nf = info["records"], max(info["nf"], 0)
+    seeds = np.empty(n, dtype=np.uint32)
+    eigs = np.empty((n, nf), dtype=np.float64)
+    got, k = ctypes.c_uint64(), ctypes.c_int()
+    _host_check(load_library().jne_dat_read(_path(path), seeds.ctypes.data, eigs.ctypes.data, n, ctypes.byref(got),
+                                            ctypes.byref(k)))
+    return seeds[:got.value], eigs[:got.value], info["model"], info["dim"], info["steps"]
+
+
+def remaining_seeds(path, model: int, dim: int, steps: int, runs: int):
+    """(remaining seeds 1..=runs not in the file, records in the file): progress.rs:10-61."""
+    L = load_library()
+    n, done = ctypes.c_uint64(), ctypes.c_uint64()
+    _host_check(L.jne_dat_remaining_seeds(_path(path), int(model), int(dim), int(steps), int(runs), None, 0,
+                                          ctypes.byref(n), ctypes.byref(done)))
+    out = np.empty(n.value, dtype=np.uint32)
+    _host_check(L.jne_dat_remaining_seeds(_path(path), int(model), int(dim), int(steps), int(runs), out.ctypes.data,
+                                          n.value, ctypes.byref(n), ctypes.byref(done)))
+    return out, done.value
+
+
+def percentiles(values, ps) -> list:
+    """Rank p*(n-1) with linear interpolation (src/simulation_analyzers.rs:4-18)."""
+    v = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+    p = np.ascontiguousarray(ps, dtype=np.float64).reshape(-1)
+    out = np.empty(len(p), dtype=np.float64)
+    _host_check(load_library().jne_percentiles(v.ctypes.data, len(v), p.ctypes.data, len(p), out.ctypes.data))
+    return out.tolist()
+
+
+class EigenvalueSimulation:
+    """The library facade of src/data_storage/simulation.rs:9-123 and
+    src/simulation_analyzers.rs:67-81 (LIBRARY_USAGE.md), on the device.
+
+    Results go to data/eigenvalues_model{m}_dim{d}_steps{T}.dat under the
+    current directory (data_dir overrides it); runs resume from that file.
+    precision selects the moment accumulation (PRECISIONS); the default is
+    the FP64 path."""
+
+    def __init__(self, model, dim: int, steps: int, num_runs: int, precision: str = "fp64", engine=None,
+                 data_dir: str = "data"):
+        self.model = JohansenModel(int(model))
+        self.dim = int(dim)
+        self.steps = int(steps)
+        self.num_runs = int(num_runs)
+        self.precision = precision
+        self.data_dir = data_dir
+        self._engine = engine
+
+    def __repr__(self):
+        return (f"EigenvalueSimulation(model={self.model!r}, dim={self.dim}, steps={self.steps}, "
+                f"num_runs={self.num_runs})")
+
+    def _run(self, quiet: bool):
+        eng = self._engine or default_engine()
+        rc = eng._lib.jne_run_simulation(eng._ctx, int(self.model), self.dim, self.steps, self.num_runs,
+                                         PRECISIONS[self.precision], _path(self.get_filename(self.model)),
+                                         1 if quiet else 0)
+        _host_check(rc)
+
+    def run_simulation(self):
+        self._run(False)
+
+    def run_simulation_quiet(self):
+        self._run(True)
+
+    def read_all_data(self):
+        """Every record of the file as [(seed, [eigenvalues])], file order (simulation.rs:48-53)."""
+        seeds, eigs, *_ = read_dat(self.get_filename(self.model))
+        return [(int(s), e.tolist()) for s, e in zip(seeds, eigs)]
+
+    def read_data(self):
+        """Records with seed <= num_runs; an error unless there are exactly num_runs (simulation.rs:55-89)."""
+        data = [r for r in self.read_all_data() if r[0] <= self.num_runs]
+        if len(data) != self.num_runs:
+            n = len(data)
+            raise JneError(JNE_ERR_FORMAT,
+                           f"Data count mismatch: expected {self.num_runs} records, found {n} records. "
+                           f"This indicates incomplete simulation data.\n"
+                           f"Solutions:\n"
+                           f"1. Use `read_all_data()` to read available data ({n} records)\n"
+                           f"2. Run `run_simulation()` to complete the simulation\n"
+                           f"3. Create a new EigenvalueSimulation with num_runs={n} to match existing data")
+        return data
+
+    def get_filename(self, model=None) -> str:
+        """data/eigenvalues_model{m}_dim{d}_steps{T}.dat; creates the folder (simulation.rs:91-122)."""
+        m = int(self.model if model is None else model)
+        os.makedirs(self.data_dir, exist_ok=True)
+        return os.path.join(self.data_dir, f"eigenvalues_model{m}_dim{self.dim}_steps{self.steps}.dat")
+
+    def _stat_percentiles(self, stat: int, ps):
+        data = self.read_data()
+        if not data:
+            return []
+        vals = [sum(e) if stat == STAT_TRACE else max(e, default=float("-inf")) for _, e in data]
+        return percentiles(vals, ps)
+
+    def calculate_trace_percentiles(self, ps):
+        """Percentiles of sum(rho) over the records (simulation_analyzers.rs:26-31,67-73)."""
+        return self._stat_percentiles(STAT_TRACE, ps)
+
+    def calculate_maxeig_percentiles(self, ps):
+        """Percentiles of max(rho) over the records (simulation_analyzers.rs:34-39,75-80)."""
+        return self._stat_percentiles(STAT_MAXEIG, ps)
 
 
 _default_engine = None

@@ -1,0 +1,154 @@
+"""GPU runs of the simulation driver, the library facade and the CLI:
+streaming to .dat, resume, the partial-record fix, parameter mismatch, the
+facade's read/percentile API (mirroring src/tests/data_storage/simulation_test.rs
+and integration/resumable.rs) and the CLI's messages."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import assert_eigs_close
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "johansen-null-eigenspectra_amd", "bin", "johansen-null-eigenspectra")
+
+
+def sim(jne, engine, tmp_path, model, dim, steps, runs, **kw):
+    return jne.EigenvalueSimulation(model, dim, steps, runs, engine=engine, data_dir=str(tmp_path / "data"), **kw)
+
+
+def test_run_simulation_streams_every_seed(jne, engine, O, tmp_path):
+    s = sim(jne, engine, tmp_path, 2, 3, 200, 3000)
+    s.run_simulation_quiet()
+    data = s.read_data()
+    seeds = np.array([r[0] for r in data], dtype=np.uint32)
+    eigs = np.array([r[1] for r in data])
+    assert sorted(seeds.tolist()) == list(range(1, 3001))
+    # the same kernels as the batch entry point: identical values
+    assert np.array_equal(eigs, engine.eigenvalues_by_seed(2, 3, 200, seeds))
+    for k in range(3):
+        assert_eigs_close(eigs[k], O.walk_eigenvalues_by_seed(2, 3, 200, int(seeds[k])), 1e-9)
+    assert jne.dat_info(s.get_filename())["has_trailer"]
+
+
+def test_resume_extends_without_duplicates(jne, engine, tmp_path, capfd):
+    s1 = sim(jne, engine, tmp_path, 4, 2, 150, 1000)
+    s1.run_simulation_quiet()
+    s2 = sim(jne, engine, tmp_path, 4, 2, 150, 2500)
+    s2.run_simulation()
+    out = capfd.readouterr().out
+    assert "Detected 1,000 completed out of 2,500 calculations, Seeds range: 1-1,000" in out
+    assert "Remaining 1,500 calculations to complete" in out
+    assert "SUCCESS: append write completed, wrote 2,500 data records for model 4, dim 2, steps 150" in out
+    assert "SUCCESS: append file validation successful" in out
+    data = s2.read_data()
+    assert sorted(r[0] for r in data) == list(range(1, 2501))
+    # a third call has nothing left to do
+    s2.run_simulation()
+    assert "SUCCESS: calculation for this model already completed, skipping" in capfd.readouterr().out
+
+
+def test_interrupted_file_resumes_cleanly(jne, engine, O, tmp_path):
+    s = sim(jne, engine, tmp_path, 1, 2, 120, 300)
+    path = s.get_filename()
+    seeds = np.arange(1, 101, dtype=np.uint32)
+    w = jne.DatWriter(path, 1, 2, 120)
+    w.append(seeds, engine.eigenvalues_by_seed(1, 2, 120, seeds))
+    w.abandon()
+    with open(path, "ab") as f:  # a record cut off by a crash
+        f.write(bytes([101, 3]) + bytes(11))
+    s.run_simulation_quiet()
+    data = s.read_data()
+    assert sorted(r[0] for r in data) == list(range(1, 301))
+    full = engine.eigenvalues_by_seed(1, 2, 120, np.arange(1, 301, dtype=np.uint32))
+    for sd, e in data:
+        assert np.array_equal(np.array(e), full[sd - 1])
+
+
+def test_incompatible_file_is_replaced(jne, engine, tmp_path, capfd):
+    s = sim(jne, engine, tmp_path, 0, 2, 90, 50)
+    w = jne.DatWriter(s.get_filename(), 0, 2, 91)  # same name, other steps (a hand-made clash)
+    w.append([1], np.zeros((1, 2)))
+    w.finish()
+    s.run_simulation()
+    out = capfd.readouterr().out
+    assert "WARNING: Existing file has incompatible parameters:" in out
+    assert "  Steps mismatch: file has steps 91, expected 90" in out
+    assert "Starting fresh calculation with correct parameters..." in out
+    assert len(s.read_data()) == 50
+
+
+def test_read_data_contract(jne, engine, tmp_path):
+    """simulation_test.rs:78-125: read_data errors until num_runs records exist."""
+    s5 = sim(jne, engine, tmp_path, 0, 2, 201, 5)
+    with pytest.raises(jne.JneError):
+        s5.read_all_data()
+    sim(jne, engine, tmp_path, 0, 2, 201, 3).run_simulation_quiet()
+    with pytest.raises(jne.JneError) as e:
+        s5.read_data()
+    assert "expected 5 records, found 3 records" in str(e.value)
+    assert "Use `read_all_data()`" in str(e.value)
+    assert len(s5.read_all_data()) == 3
+    s5.run_simulation_quiet()
+    assert len(s5.read_data()) == 5 and len(s5.read_all_data()) == 5
+    assert s5.get_filename().endswith(os.path.join("data", "eigenvalues_model0_dim2_steps201.dat"))
+
+
+def test_percentiles_and_mixed_precision_facade(jne, engine, tmp_path):
+    s = sim(jne, engine, tmp_path, 4, 5, 1000, 2000, precision="mfma_fp16x2")
+    s.run_simulation_quiet()
+    eigs = np.array([r[1] for r in s.read_data()])
+    ps = [0.5, 0.9, 0.95, 0.99]
+    assert s.calculate_trace_percentiles(ps) == pytest.approx(
+        np.percentile(eigs.sum(1), [100 * p for p in ps], method="linear"), rel=1e-12)
+    assert s.calculate_maxeig_percentiles(ps) == pytest.approx(
+        np.percentile(eigs.max(1), [100 * p for p in ps], method="linear"), rel=1e-12)
+    ref = engine.eigenvalues_by_seed(4, 5, 1000, np.arange(1, 2001, dtype=np.uint32))
+    order = np.argsort([r[0] for r in s.read_data()])
+    rmax = ref.max(1, keepdims=True)
+    assert (np.abs(eigs[order] - ref) <= 1e-5 * rmax).all()
+
+
+def run_cli(cwd, *args):
+    p = subprocess.run([CLI, *args], cwd=cwd, capture_output=True, text=True, timeout=300)
+    return p.returncode, p.stdout, p.stderr
+
+
+def test_cli_run_resume_and_quiet(tmp_path):
+    rc, out, err = run_cli(tmp_path, "--dim", "2", "--steps", "100", "--runs", "20,000", "--model", "0,4")
+    assert rc == 0, err
+    for line in ["=== Large-scale Simulation Demo ===",
+                 "  Steps: 100", "  Runs: 20,000", "  Dimension range: 2 - 2",
+                 "Simulation config: 2 dimensions, 100 steps, 20,000 runs",
+                 "Dimensions: 2, Steps: 100, Runs: 20,000",
+                 "Using model: Model 0: No intercept, no trend (supports resuming from checkpoint)",
+                 "Remaining 20,000 calculations to complete",
+                 "File pre-allocation disabled to ensure resume functionality works correctly",
+                 "SUCCESS: append write completed, wrote 20,000 data records for model 0, dim 2, steps 100",
+                 "Saved to data/eigenvalues_model0_dim2_steps100.dat",
+                 "SUCCESS: append file validation successful",
+                 "Trace for model Model 0: No intercept, no trend:",
+                 "MaxEig for model Model 4: Intercept, trend, intercept and trend not fully explained by cointegration:",
+                 "Total calculated 20,000 values",
+                 "=== Result Reading Demo ===",
+                 "Successfully read data for model Model 0: No intercept, no trend: 20,000 records",
+                 "=== All Models Status ===",
+                 "  Model 4: Intercept, trend, intercept and trend not fully explained by cointegration: 20,000 data records",
+                 "Demo completed!"]:
+        assert line in out.splitlines(), line
+    assert any(l.startswith("Simulation progress: 10,000/20,000 (50.00%) - ") for l in out.splitlines())
+    assert any(l.startswith("Simulation completed! Duration: ") for l in out.splitlines())
+    assert any(l.startswith("  Record  1: seed=") for l in out.splitlines())
+    for m in (0, 4):
+        f = tmp_path / "data" / f"eigenvalues_model{m}_dim2_steps100.dat"
+        assert f.exists()
+    # rerun: nothing left; quiet prints nothing
+    rc, out, _ = run_cli(tmp_path, "--dim", "2", "--steps", "100", "--runs", "20,000", "--model", "0")
+    assert "SUCCESS: calculation for this model already completed, skipping" in out
+    rc, out, _ = run_cli(tmp_path, "--dim", "1", "--steps", "60", "--runs", "500", "--model", "3", "--quiet",
+                         "--precision", "mfma_fp16x2")
+    assert rc == 0 and out == ""
+    assert (tmp_path / "data" / "eigenvalues_model3_dim1_steps60.dat").exists()
