@@ -207,6 +207,34 @@ int jne_dat_statistics(const char *path, uint64_t runs, int statistic, double *o
 int jne_run_simulation(jne_ctx *ctx, int model, int dim, uint32_t steps, uint64_t runs, int precision,
                        const char *path, int quiet);
 
+/* ------------------------------------------------------------------------
+ * Query server.  No counterpart in the reference, which only runs batch
+ * jobs; a query returns what its analysis step computes from a finished
+ * file: the 0.90, 0.95 and 0.99 quantiles of the trace and max-eigenvalue
+ * statistics (src/simulation_analyzers.rs:4-81) over `runs` walks.
+ * Walk w (0-based) of a query is walk seed w+1 with Philox key word `seed`,
+ * so a query's result depends only on (model, dim, steps, runs, seed,
+ * precision): the same alone and in any mix of concurrent queries.
+ * Queries are admitted in arrival order when their device footprint
+ * (16 bytes per walk) fits the HBM budget; admitted queries share `lanes`
+ * launch streams chunk by chunk (continuous batching).
+ * ------------------------------------------------------------------------ */
+#define JNE_ERR_TIMEOUT (-8)
+typedef struct jne_server jne_server;
+int jne_server_create(int device, uint64_t hbm_budget_bytes, int lanes, jne_server **out);
+/* Stops the scheduler; queries not yet waited for are dropped. */
+void jne_server_destroy(jne_server *server);
+/* Enqueue a query; returns at once with its id. */
+int jne_server_submit(jne_server *server, int model, int dim, uint32_t steps, uint64_t runs, uint32_t seed,
+                      int precision, uint64_t *id);
+/* Wait for query id (timeout_ms < 0: no limit; JNE_ERR_TIMEOUT when it
+ * runs out, the query stays pending).  quantiles[6] = trace p90, p95, p99,
+ * max-eigenvalue p90, p95, p99; latency_ms = submit to completion.  An id
+ * can be waited for once. */
+int jne_server_wait(jne_server *server, uint64_t id, int timeout_ms, double *quantiles, double *latency_ms);
+/* Walks completed and seconds with at least one launch in flight. */
+int jne_server_counters(jne_server *server, uint64_t *walks, double *busy_seconds);
+
 #ifdef __cplusplus
 }
 #endif

@@ -32,6 +32,7 @@ JNE_ERR_NOMEM = -4
 JNE_ERR_NUMERIC = -5
 JNE_ERR_IO = -6
 JNE_ERR_FORMAT = -7
+JNE_ERR_TIMEOUT = -8
 
 STAT_TRACE = 0
 STAT_MAXEIG = 1
@@ -142,6 +143,18 @@ def load_library() -> ctypes.CDLL:
         L.jne_run_simulation.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
                                          ctypes.c_uint64, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
         L.jne_run_simulation.restype = ctypes.c_int
+        L.jne_server_create.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_int, c_void_pp]
+        L.jne_server_create.restype = ctypes.c_int
+        L.jne_server_destroy.argtypes = [ctypes.c_void_p]
+        L.jne_server_destroy.restype = None
+        L.jne_server_submit.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64,
+                                        ctypes.c_uint32, ctypes.c_int, c_u64_p]
+        L.jne_server_submit.restype = ctypes.c_int
+        L.jne_server_wait.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p,
+                                      ctypes.POINTER(ctypes.c_double)]
+        L.jne_server_wait.restype = ctypes.c_int
+        L.jne_server_counters.argtypes = [ctypes.c_void_p, c_u64_p, ctypes.POINTER(ctypes.c_double)]
+        L.jne_server_counters.restype = ctypes.c_int
         _lib = L
         return L
 
@@ -377,6 +390,54 @@ def percentiles(values, ps) -> list:
     out = np.empty(len(p), dtype=np.float64)
     _host_check(load_library().jne_percentiles(v.ctypes.data, len(v), p.ctypes.data, len(p), out.ctypes.data))
     return out.tolist()
+
+
+class Server:
+    """Critical-value query server (continuous batching over `lanes` launch
+    streams, HBM-footprint admission).  wait() returns
+    (quantiles[6], latency_ms): trace p90, p95, p99, max-eigenvalue p90, p95, p99."""
+
+    QUANTILES = (0.90, 0.95, 0.99)
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 8 << 30, lanes: int = 4):
+        self._lib = load_library()
+        s = ctypes.c_void_p()
+        _host_check(self._lib.jne_server_create(int(device), int(hbm_budget_bytes), int(lanes), ctypes.byref(s)))
+        self._s = s
+
+    def submit(self, model: int, dim: int, steps: int, runs: int, seed: int, precision: str = "mfma_fp16x2") -> int:
+        i = ctypes.c_uint64()
+        _host_check(self._lib.jne_server_submit(self._s, int(model), int(dim), int(steps), int(runs), int(seed),
+                                                PRECISIONS[precision], ctypes.byref(i)))
+        return i.value
+
+    def wait(self, qid: int, timeout_ms: int = -1):
+        q = np.empty(6, dtype=np.float64)
+        lat = ctypes.c_double()
+        _host_check(self._lib.jne_server_wait(self._s, int(qid), int(timeout_ms), q.ctypes.data, ctypes.byref(lat)))
+        return q, lat.value
+
+    def counters(self):
+        w, b = ctypes.c_uint64(), ctypes.c_double()
+        _host_check(self._lib.jne_server_counters(self._s, ctypes.byref(w), ctypes.byref(b)))
+        return w.value, b.value
+
+    def close(self):
+        if getattr(self, "_s", None):
+            self._lib.jne_server_destroy(self._s)
+            self._s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 class EigenvalueSimulation:

@@ -1,0 +1,459 @@
+// jne_serve.hip -- the critical-value query server.
+//
+// A query (model, dim, steps, runs, seed, precision) asks for the 0.90, 0.95
+// and 0.99 quantiles of the trace (sum of eigenvalues) and max-eigenvalue
+// statistics over `runs` walks, with the statistic and quantile definitions
+// of src/simulation_analyzers.rs:4-40.  Walk w of a query is Philox walk
+// seed w+1 on key word `seed`, so a query's walks depend only on the query:
+// its result is the same alone and in any mix.
+//
+// Continuous batching: a scheduler thread keeps the admitted queries in a
+// round-robin ring and hands their walks out in chunks to `lanes` launch
+// lanes (each a HIP stream with its own engine context, default priority).
+// A query that arrives while others run is admitted as soon as its HBM
+// footprint (2 doubles per walk for its per-walk statistics) fits the
+// budget, and its chunks interleave with the running queries' chunks on the
+// device.  When a query's last chunk is done its statistics come back to the
+// host and a finisher thread selects the quantiles.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/jne.h"
+#include "jne_host.h"
+
+namespace {
+
+using jne_host::Clock;
+
+// consecutive walk seeds first, first+1, ... of one chunk
+__global__ void iota_seeds(uint32_t* __restrict__ out, uint32_t first, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = first + i;
+}
+
+// per-walk trace and max eigenvalue (simulation_analyzers.rs:26-39)
+__global__ void walk_statistics(const double* __restrict__ eigs, int nf, uint32_t n, double* __restrict__ trace,
+                                double* __restrict__ maxeig) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* e = eigs + (size_t)i * nf;
+  double s = 0.0, m = -1.7976931348623157e308;
+  for (int j = 0; j < nf; ++j) {
+    s += e[j];
+    m = fmax(m, e[j]);
+  }
+  trace[i] = s;
+  maxeig[i] = m;
+}
+
+constexpr uint32_t kChunkWalks = 1u << 16;
+constexpr double kQuantiles[3] = {0.90, 0.95, 0.99};
+
+// Quantiles p (all >= 0.5) of v by rank p*(n-1) with linear interpolation,
+// by successive selection (reorders v).
+void select_quantiles(std::vector<double>& v, const double* p, int np, double* out) {
+  const size_t n = v.size();
+  size_t lo_bound = 0;
+  for (int k = 0; k < np; ++k) {
+    if (n == 0) {
+      out[k] = NAN;
+      continue;
+    }
+    const double rank = p[k] * (double)(n - 1);
+    const size_t lo = (size_t)floor(rank), hi = (size_t)ceil(rank);
+    std::nth_element(v.begin() + lo_bound, v.begin() + lo, v.end());
+    const double a = v[lo];
+    double b = a;
+    if (hi != lo) b = *std::min_element(v.begin() + lo + 1, v.end());
+    const double w = rank - (double)lo;
+    out[k] = lo == hi ? a : a * (1.0 - w) + b * w;
+    lo_bound = lo;
+  }
+}
+
+struct Request {
+  uint64_t id;
+  int model, dim, precision;
+  uint32_t steps, seed;
+  uint64_t runs;
+  int nf;
+  Clock::time_point submitted, finished;
+  // device statistics: trace[runs], maxeig[runs] at arena offset
+  size_t arena_off = 0;
+  size_t bytes = 0;
+  uint64_t issued = 0, done = 0;  // walks
+  int status = 0;                 // JNE_OK or an error code
+  std::string error;
+  bool complete = false;
+  double q[6] = {0, 0, 0, 0, 0, 0};
+};
+
+struct Lane {
+  jne_ctx* ctx = nullptr;
+  hipStream_t st = nullptr;
+  uint32_t* d_seeds = nullptr;
+  double* d_eigs = nullptr;
+  hipEvent_t ev = nullptr;
+  std::shared_ptr<Request> req;  // chunk in flight
+  uint32_t walks = 0;
+};
+
+// first-fit allocator over one device buffer
+class Arena {
+ public:
+  bool alloc(size_t bytes, size_t* off) {
+    bytes = (bytes + 255) & ~size_t(255);
+    size_t at = 0;
+    for (auto it = used_.begin(); it != used_.end(); ++it) {
+      if (it->first - at >= bytes) break;
+      at = it->first + it->second;
+    }
+    if (at + bytes > cap_) return false;
+    used_[at] = bytes;
+    *off = at;
+    return true;
+  }
+  void free(size_t off) { used_.erase(off); }
+  size_t cap_ = 0;
+
+ private:
+  std::map<size_t, size_t> used_;
+};
+
+}  // namespace
+
+struct jne_server {
+  int device = 0;
+  char* arena_base = nullptr;
+  Arena arena;
+  std::vector<Lane> lanes;
+  std::mutex mu;
+  std::condition_variable cv;       // scheduler wake-up
+  std::condition_variable done_cv;  // completions
+  std::deque<std::shared_ptr<Request>> pending;
+  std::vector<std::shared_ptr<Request>> active;  // admitted, round-robin
+  size_t rr = 0;
+  std::map<uint64_t, std::shared_ptr<Request>> all;
+  std::deque<std::shared_ptr<Request>> to_finish;
+  uint64_t next_id = 1;
+  uint64_t epoch = 0;  // bumped by submissions and freed memory: wakes an idle scheduler
+  bool stop = false;
+  std::thread scheduler, finisher;
+  // counters
+  uint64_t walks_done = 0;
+  double busy_s = 0.0;
+};
+
+namespace {
+
+void fail_request(jne_server* s, const std::shared_ptr<Request>& r, int code, const std::string& msg) {
+  r->status = code;
+  r->error = msg;
+  r->complete = true;
+  r->finished = Clock::now();
+  s->done_cv.notify_all();
+}
+
+// Issues the next chunk of request r on lane L (called with s->mu held).
+int issue(jne_server* s, Lane& L, const std::shared_ptr<Request>& r) {
+  const uint32_t first = (uint32_t)(r->issued + 1);
+  const uint32_t n = (uint32_t)std::min<uint64_t>(kChunkWalks, r->runs - r->issued);
+  hipError_t e = hipSetDevice(s->device);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(iota_seeds, dim3((n + 255) / 256), dim3(256), 0, L.st, L.d_seeds, first, n);
+    e = hipGetLastError();
+  }
+  if (e != hipSuccess) return JNE_ERR_HIP;
+  int rc = jne_eigenvalues_by_seed_device(L.ctx, r->model, r->dim, r->steps, r->seed, L.d_seeds, n, r->precision,
+                                          L.d_eigs, L.st);
+  if (rc) return rc;
+  double* trace = reinterpret_cast<double*>(s->arena_base + r->arena_off) + r->issued;
+  double* maxeig = reinterpret_cast<double*>(s->arena_base + r->arena_off) + r->runs + r->issued;
+  hipLaunchKernelGGL(walk_statistics, dim3((n + 255) / 256), dim3(256), 0, L.st, L.d_eigs, r->nf, n, trace, maxeig);
+  if ((e = hipGetLastError()) != hipSuccess || (e = hipEventRecord(L.ev, L.st)) != hipSuccess) return JNE_ERR_HIP;
+  r->issued += n;
+  L.req = r;
+  L.walks = n;
+  return 0;
+}
+
+void scheduler_loop(jne_server* s) {
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto t_busy = Clock::now();
+  bool was_busy = false;
+  while (!s->stop) {
+    const uint64_t seen = s->epoch;
+    // admit pending queries whose statistics fit the budget (arrival order)
+    while (!s->pending.empty()) {
+      auto r = s->pending.front();
+      size_t off;
+      if (r->bytes > s->arena.cap_) {  // can never fit
+        s->pending.pop_front();
+        fail_request(s, r, JNE_ERR_NOMEM, "query footprint exceeds the server's HBM budget");
+        continue;
+      }
+      if (!s->arena.alloc(r->bytes, &off)) break;  // wait for memory, keep arrival order
+      r->arena_off = off;
+      s->pending.pop_front();
+      s->active.push_back(r);
+    }
+    // retire finished chunks
+    bool any_busy = false;
+    for (Lane& L : s->lanes) {
+      if (!L.req) continue;
+      const hipError_t q = hipEventQuery(L.ev);
+      if (q == hipErrorNotReady) {
+        any_busy = true;
+        continue;
+      }
+      auto r = L.req;
+      L.req.reset();
+      if (q != hipSuccess) {
+        fail_request(s, r, JNE_ERR_HIP, std::string("chunk failed: ") + hipGetErrorString(q));
+        continue;
+      }
+      r->done += L.walks;
+      s->walks_done += L.walks;
+      if (r->done == r->runs && r->status == 0) {
+        s->to_finish.push_back(r);
+        s->done_cv.notify_all();
+      }
+    }
+    // hand out chunks round-robin over the admitted queries
+    for (Lane& L : s->lanes) {
+      if (L.req) continue;
+      std::shared_ptr<Request> pick;
+      for (size_t k = 0; k < s->active.size() && !pick; ++k) {
+        auto& r = s->active[(s->rr + k) % s->active.size()];
+        if (r->status == 0 && r->issued < r->runs) {
+          pick = r;
+          s->rr = (s->rr + k + 1) % s->active.size();
+        }
+      }
+      if (!pick) break;
+      const int rc = issue(s, L, pick);
+      if (rc) {
+        fail_request(s, pick, rc, L.ctx ? jne_last_error(L.ctx) : "launch failed");
+        continue;
+      }
+      any_busy = true;
+    }
+    // drop queries that are fully issued and done or failed from the ring
+    s->active.erase(std::remove_if(s->active.begin(), s->active.end(),
+                                   [](const std::shared_ptr<Request>& r) {
+                                     return r->status != 0 || (r->issued == r->runs && r->done == r->runs);
+                                   }),
+                    s->active.end());
+    const auto now = Clock::now();
+    if (was_busy) s->busy_s += std::chrono::duration<double>(now - t_busy).count();
+    t_busy = now;
+    was_busy = any_busy;
+    // both waits release the lock; an idle scheduler sleeps until a query
+    // arrives or memory is freed (a pending query that does not fit yet must
+    // not keep the lock from the finisher that frees memory)
+    if (any_busy)
+      s->cv.wait_for(lk, std::chrono::microseconds(20));
+    else
+      s->cv.wait_for(lk, std::chrono::milliseconds(5), [&] { return s->stop || s->epoch != seen; });
+  }
+}
+
+void finisher_loop(jne_server* s) {
+  std::vector<double> tr, mx;
+  hipStream_t st = nullptr;
+  (void)hipSetDevice(s->device);
+  (void)hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  std::unique_lock<std::mutex> lk(s->mu);
+  while (true) {
+    s->done_cv.wait(lk, [&] { return s->stop || !s->to_finish.empty(); });
+    if (s->to_finish.empty() && s->stop) break;
+    auto r = s->to_finish.front();
+    s->to_finish.pop_front();
+    const double* dev = reinterpret_cast<const double*>(s->arena_base + r->arena_off);
+    lk.unlock();
+    tr.resize(r->runs);
+    mx.resize(r->runs);
+    hipError_t e = hipMemcpyAsync(tr.data(), dev, r->runs * sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(mx.data(), dev + r->runs, r->runs * sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    int code = e == hipSuccess ? 0 : JNE_ERR_HIP;
+    std::string msg = e == hipSuccess ? "" : std::string("statistics copy: ") + hipGetErrorString(e);
+    for (size_t i = 0; code == 0 && i < tr.size(); ++i)
+      if (isnan(tr[i]) || isnan(mx[i])) {
+        code = JNE_ERR_NUMERIC;
+        msg = "walk seed " + std::to_string(i + 1) + ": S11 is not positive definite";
+      }
+    double q[6];
+    if (code == 0) {
+      select_quantiles(tr, kQuantiles, 3, q);
+      select_quantiles(mx, kQuantiles, 3, q + 3);
+    }
+    lk.lock();
+    s->arena.free(r->arena_off);
+    ++s->epoch;
+    if (code) {
+      fail_request(s, r, code, msg);
+    } else {
+      memcpy(r->q, q, sizeof q);
+      r->complete = true;
+      r->finished = Clock::now();
+      s->done_cv.notify_all();
+    }
+    s->cv.notify_all();  // memory freed: the scheduler may admit more
+  }
+  lk.unlock();
+  if (st) (void)hipStreamDestroy(st);
+}
+
+}  // namespace
+
+extern "C" {
+
+int jne_server_create(int device, uint64_t hbm_budget_bytes, int lanes, jne_server** out) {
+  if (!out || lanes < 1 || lanes > 8 || hbm_budget_bytes < (1u << 20)) {
+    jne_host::set_error("jne_server_create: invalid argument (lanes 1..8, budget >= 1 MiB)");
+    return JNE_ERR_INVALID_ARG;
+  }
+  *out = nullptr;
+  jne_server* s = new jne_server();
+  s->device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipMalloc((void**)&s->arena_base, hbm_budget_bytes);
+  if (e != hipSuccess) {
+    jne_host::set_error(std::string("jne_server_create: ") + hipGetErrorString(e));
+    delete s;
+    return JNE_ERR_NOMEM;
+  }
+  s->arena.cap_ = hbm_budget_bytes;
+  s->lanes.resize(lanes);
+  for (Lane& L : s->lanes) {
+    int rc = jne_create(device, &L.ctx);
+    if (rc == JNE_OK) {
+      e = hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking);
+      if (e == hipSuccess) e = hipMalloc((void**)&L.d_seeds, kChunkWalks * sizeof(uint32_t));
+      if (e == hipSuccess) e = hipMalloc((void**)&L.d_eigs, (size_t)kChunkWalks * (JNE_MAX_DIM + 1) * sizeof(double));
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&L.ev, hipEventDisableTiming);
+      if (e != hipSuccess) rc = JNE_ERR_HIP;
+    }
+    if (rc != JNE_OK) {
+      jne_host::set_error(std::string("jne_server_create: lane setup failed: ") +
+                          (L.ctx ? hipGetErrorString(e) : jne_last_error(nullptr)));
+      jne_server_destroy(s);
+      return rc;
+    }
+  }
+  s->scheduler = std::thread(scheduler_loop, s);
+  s->finisher = std::thread(finisher_loop, s);
+  *out = s;
+  return JNE_OK;
+}
+
+void jne_server_destroy(jne_server* s) {
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+    s->cv.notify_all();
+    s->done_cv.notify_all();
+  }
+  if (s->scheduler.joinable()) s->scheduler.join();
+  if (s->finisher.joinable()) s->finisher.join();
+  (void)hipSetDevice(s->device);
+  for (Lane& L : s->lanes) {
+    if (L.st) (void)hipStreamSynchronize(L.st);
+    if (L.ev) (void)hipEventDestroy(L.ev);
+    if (L.d_eigs) (void)hipFree(L.d_eigs);
+    if (L.d_seeds) (void)hipFree(L.d_seeds);
+    if (L.st) (void)hipStreamDestroy(L.st);
+    if (L.ctx) jne_destroy(L.ctx);
+  }
+  if (s->arena_base) (void)hipFree(s->arena_base);
+  delete s;
+}
+
+int jne_server_submit(jne_server* s, int model, int dim, uint32_t steps, uint64_t runs, uint32_t seed, int precision,
+                      uint64_t* id) {
+  if (!s || !id) {
+    jne_host::set_error("jne_server_submit: NULL argument");
+    return JNE_ERR_INVALID_ARG;
+  }
+  const int nf = jne_num_eigenvalues(model, dim);
+  if (nf < 0 || dim > JNE_MAX_DIM || steps < 2 || steps > (1u << 30) || runs < 1 || runs > 0xFFFFFFFFull) {
+    jne_host::set_error("jne_server_submit: invalid model, dim, steps or runs");
+    return JNE_ERR_INVALID_ARG;
+  }
+  if (precision != JNE_PREC_FP64 && precision != JNE_PREC_MFMA_FP16X2 && precision != JNE_PREC_MFMA_BF16X2 &&
+      precision != JNE_PREC_MFMA_BF16X3 && precision != JNE_PREC_MFMA_FP32) {
+    jne_host::set_error("jne_server_submit: unsupported precision");
+    return JNE_ERR_INVALID_ARG;
+  }
+  auto r = std::make_shared<Request>();
+  r->model = model;
+  r->dim = dim;
+  r->steps = steps;
+  r->runs = runs;
+  r->seed = seed;
+  r->precision = precision;
+  r->nf = nf;
+  r->bytes = (size_t)runs * 2 * sizeof(double);
+  r->submitted = Clock::now();
+  std::lock_guard<std::mutex> lk(s->mu);
+  r->id = s->next_id++;
+  s->all[r->id] = r;
+  s->pending.push_back(r);
+  ++s->epoch;
+  s->cv.notify_all();
+  *id = r->id;
+  return JNE_OK;
+}
+
+int jne_server_wait(jne_server* s, uint64_t id, int timeout_ms, double* quantiles, double* latency_ms) {
+  if (!s) {
+    jne_host::set_error("jne_server_wait: NULL server");
+    return JNE_ERR_INVALID_ARG;
+  }
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->all.find(id);
+  if (it == s->all.end()) {
+    jne_host::set_error("jne_server_wait: unknown request id");
+    return JNE_ERR_INVALID_ARG;
+  }
+  auto r = it->second;
+  auto ready = [&] { return r->complete; };
+  if (timeout_ms < 0)
+    s->done_cv.wait(lk, ready);
+  else if (!s->done_cv.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready))
+    return JNE_ERR_TIMEOUT;
+  s->all.erase(id);
+  if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r->finished - r->submitted).count();
+  if (r->status) {
+    jne_host::set_error(r->error);
+    return r->status;
+  }
+  if (quantiles) memcpy(quantiles, r->q, sizeof r->q);
+  return JNE_OK;
+}
+
+int jne_server_counters(jne_server* s, uint64_t* walks, double* busy_seconds) {
+  if (!s) return JNE_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (walks) *walks = s->walks_done;
+  if (busy_seconds) *busy_seconds = s->busy_s;
+  return JNE_OK;
+}
+
+}  // extern "C"
