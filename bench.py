@@ -74,6 +74,58 @@ def cpu_baseline(model, dim, steps, threads, target_s=12.0):
     return n / dt, n, dt, eig
 
 
+def serving_requests(n, rng):
+    """Critical-value queries: model U{0..4}, dim U{1..12}, T log-uniform in
+    [100, 10,000], runs log-uniform in [10^4, 10^6], each with its own seed."""
+    out = []
+    for _ in range(n):
+        out.append((int(rng.integers(0, 5)), int(rng.integers(1, 13)),
+                    int(round(10 ** rng.uniform(2.0, 4.0))), int(round(10 ** rng.uniform(4.0, 6.0))),
+                    int(rng.integers(1, 2**31))))
+    return out
+
+
+def serve_at_rate(server, reqs, rate, precision, rng):
+    """Submits reqs with Poisson arrivals at `rate` per second; returns
+    per-query latencies (ms, submit to completion) and the wall time from the
+    first submission to the last completion."""
+    import numpy as np
+
+    gaps = rng.exponential(1.0 / rate, len(reqs))
+    t0 = time.perf_counter()
+    due = t0
+    ids = []
+    for gap, r in zip(gaps, reqs):
+        due += gap
+        wait = due - time.perf_counter()
+        if wait > 0:
+            time.sleep(wait)
+        ids.append(server.submit(*r, precision=precision))
+    lat = np.array([server.wait(i)[1] for i in ids])
+    return lat, time.perf_counter() - t0
+
+
+def serving_sweep(pkg, rates, n_req, precision, seed=2024):
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    reqs = serving_requests(n_req, rng)
+    walk_steps = sum(r[2] * r[3] for r in reqs)
+    walks = sum(r[3] for r in reqs)
+    out = {"precision": precision, "requests_per_rate": n_req,
+           "mix": "model U{0..4}, dim U{1..12}, T log-U[100,1e4], runs log-U[1e4,1e6], own seed",
+           "walks_per_rate": walks, "rates": []}
+    with pkg.Server(0, hbm_budget_bytes=8 << 30, lanes=4) as server:
+        server.wait(server.submit(4, 12, 1000, 70000, 1, precision=precision))  # warm the lanes
+        for rate in rates:
+            lat, wall = serve_at_rate(server, reqs, rate, precision, np.random.default_rng(seed + int(rate)))
+            out["rates"].append({"offered_req_per_s": rate, "p50_ms": float(np.percentile(lat, 50)),
+                                 "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
+                                 "walks_per_s": walks / wall, "walk_steps_per_s": walk_steps / wall,
+                                 "achieved_req_per_s": len(reqs) / wall})
+    return out
+
+
 def time_path(pkg, eng, prec_name, model, dim, T, seeds, out, n, steps, warmup, stream, torch):
     """Runs warmup + steps passes; returns (walks/s, s/step, moments kernel (ms, launches, walks), eigen ms)."""
     prec = pkg.PRECISIONS[prec_name]
@@ -112,6 +164,10 @@ def main() -> int:
     ap.add_argument("--check-walks", type=int, default=10_000, help="walks compared between the two paths")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cores)")
+    ap.add_argument("--serve-rates", default="10,30,100,300,1000",
+                    help="Poisson query rates (req/s) for the serving sweep ('' to skip)")
+    ap.add_argument("--serve-requests", type=int, default=100, help="queries per rate")
+    ap.add_argument("--serve-precision", default="auto", help="precision of the served queries")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run one device copy of this many MiB (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -235,6 +291,15 @@ def main() -> int:
         fp64_head = ref if args.compare == "fp64" else None
     else:
         fp64_head = head if args.precision == "fp64" else None
+
+    if args.serve_rates:
+        rates = [float(r) for r in args.serve_rates.split(",")]
+        sv = serving_sweep(pkg, rates, args.serve_requests, args.serve_precision)
+        # the same query sequence on the FP64 path at the highest rate: both saturated
+        top = serving_sweep(pkg, [rates[-1]], args.serve_requests, "fp64")["rates"][0]
+        sv["fp64_at_top_rate"] = top
+        sv["saturated_mixed_vs_fp64"] = sv["rates"][-1]["walk_steps_per_s"] / top["walk_steps_per_s"]
+        line["serving"] = sv
 
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)

@@ -43,6 +43,7 @@ PREC_MFMA_FP16X2 = 2
 PREC_MFMA_BF16X2 = 3
 PREC_MFMA_BF16X3 = 4
 PREC_MFMA_FP32 = 5
+PREC_AUTO = 6
 PRECISIONS = {
     "fp64": PREC_FP64,
     "fp32_kahan": PREC_FP32_KAHAN,
@@ -50,6 +51,7 @@ PRECISIONS = {
     "mfma_bf16x2": PREC_MFMA_BF16X2,
     "mfma_bf16x3": PREC_MFMA_BF16X3,
     "mfma_fp32": PREC_MFMA_FP32,
+    "auto": PREC_AUTO,
 }
 
 MAX_DIM = 12

@@ -121,6 +121,7 @@ int check_args(jne_ctx* ctx, int model, int dim, uint32_t steps, int precision) 
     case JNE_PREC_MFMA_BF16X2:
     case JNE_PREC_MFMA_BF16X3:
     case JNE_PREC_MFMA_FP32:
+    case JNE_PREC_AUTO:
       return JNE_OK;
     case JNE_PREC_FP32_KAHAN:
       return set_err(ctx, JNE_ERR_UNSUPPORTED, "precision %d (FP32 Kahan VALU) is not available in this build",
@@ -168,8 +169,12 @@ hipError_t timed_launch(jne_ctx* ctx, int kernel, uint64_t walks, hipStream_t st
 }
 
 // Enqueue the walk path for n walks (device pointers) on stream st.
+// JNE_PREC_AUTO: the MFMA pass from this dim up (measured crossover, model 2/4, T = 10^3..10^4)
+constexpr int kAutoMfmaMinDim = 11;
+
 int run_walks(jne_ctx* ctx, int model, int dim, uint32_t steps, uint32_t rng_stream, const uint32_t* d_seeds,
               const double* d_z, size_t n, int precision, double* d_out, hipStream_t st) {
+  if (precision == JNE_PREC_AUTO) precision = dim >= kAutoMfmaMinDim ? JNE_PREC_MFMA_FP16X2 : JNE_PREC_FP64;
   const int nf = jne::num_eigenvalues(model, dim);
   const size_t mc = (size_t)jne::moment_count(dim);
   const size_t chunk = std::min(n, kChunkWalks);

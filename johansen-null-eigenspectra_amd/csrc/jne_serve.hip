@@ -397,7 +397,7 @@ int jne_server_submit(jne_server* s, int model, int dim, uint32_t steps, uint64_
     return JNE_ERR_INVALID_ARG;
   }
   if (precision != JNE_PREC_FP64 && precision != JNE_PREC_MFMA_FP16X2 && precision != JNE_PREC_MFMA_BF16X2 &&
-      precision != JNE_PREC_MFMA_BF16X3 && precision != JNE_PREC_MFMA_FP32) {
+      precision != JNE_PREC_MFMA_BF16X3 && precision != JNE_PREC_MFMA_FP32 && precision != JNE_PREC_AUTO) {
     jne_host::set_error("jne_server_submit: unsupported precision");
     return JNE_ERR_INVALID_ARG;
   }

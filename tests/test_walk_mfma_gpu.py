@@ -133,3 +133,12 @@ def test_fp32_kahan_unsupported(engine, jne):
     with pytest.raises(jne.JneError) as e:
         engine.eigenvalues_by_seed(2, 3, 100, [1], precision=jne.PRECISIONS["fp32_kahan"])
     assert e.value.code == jne.JNE_ERR_UNSUPPORTED
+
+
+def test_auto_precision_picks_the_faster_path(engine, jne):
+    """auto runs the MFMA pass for dims 11 and 12 and the FP64 pass below."""
+    seeds = np.arange(1, 33, dtype=np.uint32)
+    for dim, same_as in [(12, "mfma_fp16x2"), (11, "mfma_fp16x2"), (10, "fp64"), (3, "fp64")]:
+        a = engine.eigenvalues_by_seed(4, dim, 700, seeds, precision=jne.PRECISIONS["auto"])
+        b = engine.eigenvalues_by_seed(4, dim, 700, seeds, precision=jne.PRECISIONS[same_as])
+        assert np.array_equal(a, b), dim
