@@ -32,10 +32,11 @@ if ROOT not in sys.path:
 METRIC = "eigen-solved walks/s @dim12 T=10k model4 vs FP64 GPU; p50/p99 query latency"
 # MI355X published dense peaks, TFLOP/s
 PEAK_TFLOPS = {"fp64": 78.6, "mfma_fp16x2": 2500.0, "mfma_bf16x2": 2500.0, "mfma_bf16x3": 2500.0,
-               "mfma_fp32": 157.3}
+               "mfma_fp32": 157.3, "fp32_kahan": 157.3}
 DTYPE = {"fp64": "f64", "mfma_fp16x2": "fp16 hi/lo MFMA (fp32 accumulate) + f64",
          "mfma_bf16x2": "bf16 hi/lo MFMA (fp32 accumulate) + f64",
-         "mfma_bf16x3": "bf16 hi/mid/lo MFMA (fp32 accumulate) + f64", "mfma_fp32": "fp32 MFMA + f64"}
+         "mfma_bf16x3": "bf16 hi/mid/lo MFMA (fp32 accumulate) + f64", "mfma_fp32": "fp32 MFMA + f64",
+         "fp32_kahan": "fp32 products, Kahan-compensated fp32 sums + f64 state"}
 # MFMA operations issued per 16-step chunk of one walk (32x32x16 products)
 MFMA_PER_CHUNK = {"mfma_fp16x2": 3, "mfma_bf16x2": 3, "mfma_bf16x3": 6}
 
@@ -126,6 +127,44 @@ def serving_sweep(pkg, rates, n_req, precision, seed=2024):
     return out
 
 
+def precision_ladder(pkg, eng, model, dim, T, n, oracle_walks=64):
+    """Each reduced precision against the FP64 path on walks 1..n: per-eigenvalue
+    max/mean relative error (floor rule) and the shift of the trace and
+    max-eigenvalue quantiles; the FP64 path itself against the CPU oracle on
+    the first oracle_walks walks."""
+    import numpy as np
+
+    import oracle
+
+    seeds = np.arange(1, n + 1, dtype=np.uint32)
+    ref = eng.eigenvalues_by_seed(model, dim, T, seeds)
+    cpu = oracle.walks_by_seed_c(model, dim, T, seeds[:oracle_walks], threads=min(16, os.cpu_count() or 1))
+    anchor = np.abs(ref[:oracle_walks] - cpu) / (np.abs(cpu) + 1e-13 * cpu.max(axis=1, keepdims=True))
+    ps = [0.90, 0.95, 0.99]
+    rng = np.random.default_rng(0)
+
+    def boot_se(x):
+        q = np.array([np.quantile(x[rng.integers(0, len(x), len(x))], ps) for _ in range(200)])
+        return q.std(axis=0)
+
+    stats_ref = {"trace": ref.sum(1), "maxeig": ref.max(1)}
+    out = {"workload": f"model {model}, dim {dim}, T={T:,}, walks 1..{n:,}",
+           "fp64_vs_cpu_oracle_max_rel": float(anchor.max()),
+           "mc_standard_error": {k: (boot_se(v) / np.quantile(v, ps)).tolist() for k, v in stats_ref.items()},
+           "precisions": {}}
+    for name in ["mfma_bf16x2", "mfma_fp16x2", "mfma_bf16x3", "mfma_fp32", "fp32_kahan"]:
+        got = eng.eigenvalues_by_seed(model, dim, T, seeds, precision=pkg.PRECISIONS[name])
+        rel = relative_floor(got, ref)
+        row = {"max_rel": float(rel.max()), "mean_rel": float(rel.mean()),
+               "per_eigenvalue_max_rel": rel.max(axis=0).tolist(),
+               "per_eigenvalue_mean_rel": rel.mean(axis=0).tolist()}
+        for k, f in [("trace", lambda e: e.sum(1)), ("maxeig", lambda e: e.max(1))]:
+            qa, qb = np.quantile(f(got), ps), np.quantile(stats_ref[k], ps)
+            row[f"{k}_quantile_rel_shift"] = ((qa - qb) / qb).tolist()
+        out["precisions"][name] = row
+    return out
+
+
 def time_path(pkg, eng, prec_name, model, dim, T, seeds, out, n, steps, warmup, stream, torch):
     """Runs warmup + steps passes; returns (walks/s, s/step, moments kernel (ms, launches, walks), eigen ms)."""
     prec = pkg.PRECISIONS[prec_name]
@@ -168,6 +207,8 @@ def main() -> int:
                     help="Poisson query rates (req/s) for the serving sweep ('' to skip)")
     ap.add_argument("--serve-requests", type=int, default=100, help="queries per rate")
     ap.add_argument("--serve-precision", default="auto", help="precision of the served queries")
+    ap.add_argument("--ladder", type=int, default=0,
+                    help="print the precision ladder on this many walks instead of timing")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run one device copy of this many MiB (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -191,6 +232,10 @@ def main() -> int:
     torch.cuda.init()
     dev = torch.device("cuda:0")
     eng = pkg.Engine(0)
+    if args.ladder:
+        print(json.dumps(precision_ladder(pkg, eng, model, dim, T, args.ladder)), flush=True)
+        eng.close()
+        return 0
     seeds = torch.arange(1, n + 1, dtype=torch.int32, device=dev)  # uint32 bit pattern
     out = torch.empty(n * nf, dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream(dev).cuda_stream
@@ -215,6 +260,7 @@ def main() -> int:
     achieved = fpw * walks_per_launch / (avg_ms * 1e-3) / 1e12
     peak = PEAK_TFLOPS[args.precision]
     kernel = (f"walk_moments_fp64<{dim}>" if args.precision == "fp64"
+              else f"walk_moments_fp64<{dim}, kahan>" if args.precision == "fp32_kahan"
               else f"walk_moments_mfma<{dim}, {args.precision}>")
 
     # HBM bytes per launch from the committed counter runs (per-walk figures,

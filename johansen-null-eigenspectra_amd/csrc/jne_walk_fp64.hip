@@ -49,10 +49,50 @@ namespace jne {
 // The walk index is the fastest LDS dimension (odd pitch) so that the lanes
 // of a wave read from different banks.
 // ---------------------------------------------------------------------------
-template <int D, bool FROM_NORMALS>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(waves_for_dim(D), waves_for_dim(D))))
+// Gram accumulator: FP64 fma, or (the compensated FP32 variant) an FP32
+// product added with Kahan's compensated summation.  Both take 8 bytes.
+template <bool KAHAN>
+struct GramAcc;
+
+template <>
+struct GramAcc<false> {
+  using Real = double;
+  double s;
+  __device__ __forceinline__ void zero() { s = 0.0; }
+  __device__ __forceinline__ void add_prod(double a, double b) { s = fma(a, b, s); }
+  __device__ __forceinline__ void add(double x) { s += x; }
+  __device__ __forceinline__ double value() const { return s; }
+};
+
+template <>
+struct GramAcc<true> {
+  using Real = float;
+  float s, c;  // sum and the negated low part lost so far
+  __device__ __forceinline__ void zero() { s = c = 0.0f; }
+  __device__ __forceinline__ void add(float x) {
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  __device__ __forceinline__ void add_prod(float a, float b) { add(a * b); }
+  __device__ __forceinline__ double value() const { return (double)s - (double)c; }
+};
+
+// The compensated variant keeps an FP32 copy of the state beside the FP64 one
+// and needs more registers: one wave per SIMD fewer where the FP64 target is
+// already tight.
+constexpr int kahan_waves_for_dim(int d) {
+  return (d == 8 || d >= 11) ? 1 : (d == 6 || d == 10) ? 2 : waves_for_dim(d);
+}
+
+template <int D, bool FROM_NORMALS, bool KAHAN>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KAHAN ? kahan_waves_for_dim(D) : waves_for_dim(D),
+                                                                   KAHAN ? kahan_waves_for_dim(D) : waves_for_dim(D))))
 void walk_moments_fp64(const uint32_t* __restrict__ seeds, const double* __restrict__ zin, uint32_t rng_stream,
                        uint32_t T, uint32_t n_walks, double* __restrict__ acc, size_t ld) {
+  using Acc = GramAcc<KAHAN>;
+  using Real = typename Acc::Real;
   using Sh = WalkShape<D>;
   constexpr int L = Sh::L;
   constexpr int DP = Sh::DP;
@@ -90,19 +130,19 @@ void walk_moments_fp64(const uint32_t* __restrict__ seeds, const double* __restr
   }
 
   double b[DP];
-  double geb[R][DP];
-  double gbb[R][H + 1];
-  double mom[6][R];
+  Acc geb[R][DP];
+  Acc gbb[R][H + 1];
+  Acc mom[6][R];
 #pragma unroll
   for (int m = 0; m < DP; ++m) b[m] = 0.0;
 #pragma unroll
   for (int k = 0; k < R; ++k) {
 #pragma unroll
-    for (int m = 0; m < DP; ++m) geb[k][m] = 0.0;
+    for (int m = 0; m < DP; ++m) geb[k][m].zero();
 #pragma unroll
-    for (int m = 0; m <= H; ++m) gbb[k][m] = 0.0;
+    for (int m = 0; m <= H; ++m) gbb[k][m].zero();
 #pragma unroll
-    for (int p = 0; p < 6; ++p) mom[p][k] = 0.0;
+    for (int p = 0; p < 6; ++p) mom[p][k].zero();
   }
 
   // src/johansen_statistics.rs:70 and src/rng_matrix.rs:137: sqrt(1/steps)
@@ -169,18 +209,26 @@ void walk_moments_fp64(const uint32_t* __restrict__ seeds, const double* __restr
         bnk[k] = b[k] + w[k * GP];  // B_{t+1} = B_t + z*sqrt(dt) (rng_matrix.rs:138-140)
         e[k] = bnk[k] - b[k];  // dbm = bm_current - bm_previous (johansen_statistics.rs:80-82)
       }
+      // the Gram operands in the accumulator's precision (FP32 copies of the
+      // FP64 state for the compensated variant)
+      Real bq[DP], eq[R];
+#pragma unroll
+      for (int m = 0; m < DP; ++m) bq[m] = (Real)b[m];
+#pragma unroll
+      for (int k = 0; k < R; ++k) eq[k] = (Real)e[k];
+      const Real uq = (Real)u, u2q = (Real)u2;
 #pragma unroll
       for (int k = 0; k < R; ++k) {
 #pragma unroll
-        for (int m = 0; m < DP; ++m) geb[k][m] = fma(e[k], b[m], geb[k][m]);
+        for (int m = 0; m < DP; ++m) geb[k][m].add_prod(eq[k], bq[m]);
 #pragma unroll
-        for (int dl = 0; dl <= H; ++dl) gbb[k][dl] = fma(b[k], b[(k + dl) % DP], gbb[k][dl]);
-        mom[0][k] += b[k];
-        mom[1][k] = fma(u, b[k], mom[1][k]);
-        mom[2][k] = fma(u2, b[k], mom[2][k]);
-        mom[3][k] += e[k];
-        mom[4][k] = fma(u, e[k], mom[4][k]);
-        mom[5][k] = fma(u2, e[k], mom[5][k]);
+        for (int dl = 0; dl <= H; ++dl) gbb[k][dl].add_prod(bq[k], bq[(k + dl) % DP]);
+        mom[0][k].add(bq[k]);
+        mom[1][k].add_prod(uq, bq[k]);
+        mom[2][k].add_prod(u2q, bq[k]);
+        mom[3][k].add(eq[k]);
+        mom[4][k].add_prod(uq, eq[k]);
+        mom[5][k].add_prod(u2q, eq[k]);
       }
 #pragma unroll
       for (int k = 0; k < R; ++k) b[k] = bnk[k];
@@ -200,18 +248,18 @@ void walk_moments_fp64(const uint32_t* __restrict__ seeds, const double* __restr
 #pragma unroll
     for (int m = 0; m < DP; ++m) {
       const int cj = (rot + m) % DP;
-      if (cj < D) acc[(size_t)(D * D + ci * D + cj) * ld + walk] = geb[k][m];
+      if (cj < D) acc[(size_t)(D * D + ci * D + cj) * ld + walk] = geb[k][m].value();
     }
 #pragma unroll
     for (int dl = 0; dl <= H; ++dl) {
       const int cj = (rot + k + dl) % DP;
       if (cj < D) {
-        acc[(size_t)(ci * D + cj) * ld + walk] = gbb[k][dl];
-        acc[(size_t)(cj * D + ci) * ld + walk] = gbb[k][dl];
+        acc[(size_t)(ci * D + cj) * ld + walk] = gbb[k][dl].value();
+        acc[(size_t)(cj * D + ci) * ld + walk] = gbb[k][dl].value();
       }
     }
 #pragma unroll
-    for (int p = 0; p < 6; ++p) acc[(size_t)(2 * D * D + p * D + ci) * ld + walk] = mom[p][k];
+    for (int p = 0; p < 6; ++p) acc[(size_t)(2 * D * D + p * D + ci) * ld + walk] = mom[p][k].value();
   }
 }
 
@@ -456,18 +504,25 @@ __global__ __launch_bounds__(64) void walk_eigen_fp64(const double* __restrict__
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <int D>
-static hipError_t launch_moments_d(const uint32_t* seeds, const double* zin, uint32_t rng_stream, uint32_t T,
-                                   uint32_t n, double* acc, size_t ld, hipStream_t st) {
+template <int D, bool KAHAN>
+static hipError_t launch_moments_dk(const uint32_t* seeds, const double* zin, uint32_t rng_stream, uint32_t T,
+                                    uint32_t n, double* acc, size_t ld, hipStream_t st) {
   constexpr int GPW = WalkShape<D>::GPW;
   const unsigned grid = (unsigned)((n + GPW - 1) / GPW);
   if (zin != nullptr)
-    hipLaunchKernelGGL((walk_moments_fp64<D, true>), dim3(grid), dim3(64), 0, st, seeds, zin, rng_stream, T, n,
-                       acc, ld);
+    hipLaunchKernelGGL((walk_moments_fp64<D, true, KAHAN>), dim3(grid), dim3(64), 0, st, seeds, zin, rng_stream, T,
+                       n, acc, ld);
   else
-    hipLaunchKernelGGL((walk_moments_fp64<D, false>), dim3(grid), dim3(64), 0, st, seeds, zin, rng_stream, T, n,
-                       acc, ld);
+    hipLaunchKernelGGL((walk_moments_fp64<D, false, KAHAN>), dim3(grid), dim3(64), 0, st, seeds, zin, rng_stream,
+                       T, n, acc, ld);
   return hipGetLastError();
+}
+
+template <int D>
+static hipError_t launch_moments_d(bool kahan, const uint32_t* seeds, const double* zin, uint32_t rng_stream,
+                                   uint32_t T, uint32_t n, double* acc, size_t ld, hipStream_t st) {
+  return kahan ? launch_moments_dk<D, true>(seeds, zin, rng_stream, T, n, acc, ld, st)
+               : launch_moments_dk<D, false>(seeds, zin, rng_stream, T, n, acc, ld, st);
 }
 
 template <int MODEL, int D>
@@ -478,12 +533,12 @@ static hipError_t launch_eigen_md(const double* acc, size_t ld, uint32_t T, uint
   return hipGetLastError();
 }
 
-hipError_t launch_walk_moments_fp64(int d, const uint32_t* seeds, const double* zin, uint32_t rng_stream,
+hipError_t launch_walk_moments_fp64(int d, bool kahan, const uint32_t* seeds, const double* zin, uint32_t rng_stream,
                                     uint32_t T, uint32_t n, double* acc, size_t ld, hipStream_t st) {
   switch (d) {
 #define JNE_CASE(DD) \
   case DD:           \
-    return launch_moments_d<DD>(seeds, zin, rng_stream, T, n, acc, ld, st);
+    return launch_moments_d<DD>(kahan, seeds, zin, rng_stream, T, n, acc, ld, st);
     JNE_CASE(1) JNE_CASE(2) JNE_CASE(3) JNE_CASE(4) JNE_CASE(5) JNE_CASE(6)
     JNE_CASE(7) JNE_CASE(8) JNE_CASE(9) JNE_CASE(10) JNE_CASE(11) JNE_CASE(12)
 #undef JNE_CASE

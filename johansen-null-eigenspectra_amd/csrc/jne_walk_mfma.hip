@@ -190,6 +190,19 @@ __device__ __forceinline__ float partner_half(float v, int h) {
   return __uint_as_float(h ? r[0] : r[1]);
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin by index, and each XCD has
+// its own L2.  With one walk per workgroup, walks b and b+1 would run on
+// different XCDs and their 8-byte stores into one 32-byte sector of the
+// entry-major output would reach HBM as separate partial-sector writes.  This
+// bijection gives XCD x the contiguous walk range [start_x, start_x + count_x)
+// instead, so neighbouring walks share an L2 and their stores merge there.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_walk(uint32_t b, uint32_t n) {
+  const uint32_t q = n / kXcds, r = n % kXcds;
+  const uint32_t x = b % kXcds, i = b / kXcds;
+  return x * q + (x < r ? x : r) + i;
+}
+
 template <int D, int PREC, bool FROM_NORMALS>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(JNE_MFMA_WAVES)))
 void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restrict__ zin, uint32_t rng_stream,
@@ -212,8 +225,8 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
   const int l = threadIdx.x;
   const int row = l & 31;  // also the accumulator column
   const int h = l >> 5;
-  const size_t walk = blockIdx.x;
-  if (walk >= n_walks) return;  // whole wave exits together
+  if (blockIdx.x >= n_walks) return;  // whole wave exits together
+  const size_t walk = xcd_walk(blockIdx.x, n_walks);
   uint32_t seed = 0;
   if constexpr (!FROM_NORMALS) seed = seeds[walk];
   const double* zw = FROM_NORMALS ? zin + walk * (size_t)T * D : nullptr;

@@ -1,10 +1,12 @@
-"""GPU parity of the mixed-precision (MFMA) moment pass, through the C ABI.
+"""GPU parity of the reduced-precision moment passes, through the C ABI.
 
 The mixed path draws the same Philox increments as the FP64 path and keeps the
 state B and every per-segment term in FP64; only the within-segment Gram is
 accumulated on the matrix cores (FP32 accumulate, flushed to FP64 every 64
-steps).  Its eigenvalues are compared with the oracle / the FP64 path on the
-same seeds.
+steps).  The compensated FP32 VALU variant (fp32_kahan) runs the FP64 pass
+with FP32 products of the FP64 state added into Kahan-compensated FP32 sums.
+Their eigenvalues are compared with the oracle / the FP64 path on the same
+seeds.
 
 Two measures:
 
@@ -12,7 +14,12 @@ Two measures:
   |d_i| <= 1e-5 |rho_i|, or <= 1e-5 rho_max for an entry below 1e-5 rho_max
   (the zero eigenvalue of models 1 and 3).  Checked at model 4, dim 12,
   T = 10,000 for fp16x2, bf16x3 and fp32 (measured worst case 5e-6 over
-  10,000 walks for fp16x2).
+  10,000 walks for fp16x2).  The compensated FP32 VALU variant is held to
+  5e-4: Kahan summation removes the error of the sums, but each product of
+  the FP32-rounded O(1) state still carries a 2^-24 relative error, which
+  the Schur complements amplify in the smallest eigenvalue (measured
+  1.3e-4 over 2,000 walks; a CPU emulation of the same arithmetic gives
+  3.3e-5 over 64 walks, against 1.6e-3 with plain FP32 sums).
 * batch-normwise, for the sweep over models, dims and lengths:
   max_i |d_i| <= TOL * max(rho_max(walk), median over the batch of rho_max).
   The batch floor keeps walks whose whole spectrum is tiny (dim 1, where a
@@ -27,9 +34,14 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-TOL_LONG = {"mfma_fp16x2": 1e-5, "mfma_bf16x3": 1e-5, "mfma_fp32": 1e-5, "mfma_bf16x2": 1e-4}
-TOL_SHORT = {"mfma_fp16x2": 2e-3, "mfma_bf16x3": 2e-3, "mfma_fp32": 2e-3, "mfma_bf16x2": 5e-2}
+TOL_LONG = {"mfma_fp16x2": 1e-5, "mfma_bf16x3": 1e-5, "mfma_fp32": 1e-5, "mfma_bf16x2": 1e-4,
+            "fp32_kahan": 1e-5}
+TOL_SHORT = {"mfma_fp16x2": 2e-3, "mfma_bf16x3": 2e-3, "mfma_fp32": 2e-3, "mfma_bf16x2": 5e-2,
+             "fp32_kahan": 2e-3}
 PRECS = list(TOL_LONG)
+# relative-with-floor bound at the headline configuration (None: normwise only)
+REL_HEADLINE = {"mfma_fp16x2": 1e-5, "mfma_bf16x3": 1e-5, "mfma_fp32": 1e-5, "mfma_bf16x2": None,
+                "fp32_kahan": 5e-4}
 
 
 def normwise(got, ref):
@@ -75,19 +87,19 @@ def test_mixed_matches_oracle_headline(engine, jne, O, prec):
     got = engine.eigenvalues_by_seed(4, 12, 10000, seeds, precision=jne.PRECISIONS[prec])
     err = normwise(got, ref).max()
     assert err <= TOL_LONG[prec], f"{prec}: {err:.2e}"
-    if prec != "mfma_bf16x2":
+    if REL_HEADLINE[prec] is not None:
         rel = relative_floor(got, ref).max()
-        assert rel <= 1e-5, f"{prec}: relative {rel:.2e}"
+        assert rel <= REL_HEADLINE[prec], f"{prec}: relative {rel:.2e}"
 
 
-@pytest.mark.parametrize("prec", ["mfma_fp16x2", "mfma_bf16x3"])
+@pytest.mark.parametrize("prec", ["mfma_fp16x2", "mfma_bf16x3", "fp32_kahan"])
 def test_mixed_relative_tolerance_headline_batch(engine, jne, prec):
     """The stated tolerance on 2,000 walks of the headline configuration."""
     seeds = np.arange(1, 2001, dtype=np.uint32)
     ref = engine.eigenvalues_by_seed(4, 12, 10000, seeds)
     got = engine.eigenvalues_by_seed(4, 12, 10000, seeds, precision=jne.PRECISIONS[prec])
     rel = relative_floor(got, ref)
-    assert rel.max() <= 1e-5, f"{prec}: max relative {rel.max():.2e}"
+    assert rel.max() <= REL_HEADLINE[prec], f"{prec}: max relative {rel.max():.2e}"
 
 
 @pytest.mark.parametrize("model,dim,steps", [(2, 5, 5000), (1, 7, 3001), (3, 12, 2047), (0, 9, 1100)])
@@ -129,10 +141,15 @@ def test_mixed_deterministic_and_sorted(engine, jne):
     assert (np.diff(a, axis=1) <= 0).all() and (a >= 0).all()
 
 
-def test_fp32_kahan_unsupported(engine, jne):
-    with pytest.raises(jne.JneError) as e:
-        engine.eigenvalues_by_seed(2, 3, 100, [1], precision=jne.PRECISIONS["fp32_kahan"])
-    assert e.value.code == jne.JNE_ERR_UNSUPPORTED
+def test_fp32_kahan_all_dims(engine, jne):
+    """Every dim's compensated kernel (each has its own register budget)."""
+    p = jne.PRECISIONS["fp32_kahan"]
+    seeds = np.arange(1, 65, dtype=np.uint32)
+    for dim in range(1, 13):
+        ref = engine.eigenvalues_by_seed(2, dim, 2500, seeds)
+        got = engine.eigenvalues_by_seed(2, dim, 2500, seeds, precision=p)
+        err = normwise(got, ref).max()
+        assert err <= TOL_LONG["fp32_kahan"], f"dim {dim}: {err:.2e}"
 
 
 def test_auto_precision_picks_the_faster_path(engine, jne):
