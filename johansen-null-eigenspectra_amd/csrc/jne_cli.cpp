@@ -4,7 +4,7 @@
 //
 // Added flags (no counterpart in the reference): --precision <name> picks the
 // moment accumulation (fp64 default; auto, mfma_fp16x2, mfma_bf16x3,
-// mfma_bf16x2, mfma_fp32) and --device <int> the HIP device.
+// mfma_bf16x2, mfma_fp32, fp32_kahan) and --device <int> the HIP device.
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -53,7 +53,7 @@ void print_help(const char* prog) {
   printf("  --model <list>       comma separated list of model numbers to compute (default: 0,1,2,3,4)\n");
   printf("  --quiet              suppress progress output\n");
   printf("  --precision <name>   moment accumulation: fp64 (default), auto, mfma_fp16x2, mfma_bf16x3, "
-         "mfma_bf16x2, mfma_fp32\n");
+         "mfma_bf16x2, mfma_fp32, fp32_kahan\n");
   printf("  --device <int>       HIP device (default: 0)\n");
   printf("  -h, --help           show this help message\n");
   printf("  -v, --version        show version information\n\n");
@@ -202,6 +202,7 @@ int parse(int argc, char** argv, CliArgs* a) {
       else if (p == "mfma_bf16x3") a->precision = JNE_PREC_MFMA_BF16X3;
       else if (p == "mfma_bf16x2") a->precision = JNE_PREC_MFMA_BF16X2;
       else if (p == "mfma_fp32") a->precision = JNE_PREC_MFMA_FP32;
+      else if (p == "fp32_kahan") a->precision = JNE_PREC_FP32_KAHAN;
       else if (p == "auto") a->precision = JNE_PREC_AUTO;
       else {
         fprintf(stderr, "Error: unknown precision '%s'\n", p.c_str());
