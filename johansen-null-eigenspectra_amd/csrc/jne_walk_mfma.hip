@@ -64,9 +64,11 @@ constexpr int kRowC = 12;      // first row of the c block
 constexpr int kRowDet = 24;    // rows 24, 25, 26: 1, k, k(k-1)/2
 
 // The FP64 Gram entries the output needs, compact in LDS:
-//   column c in [12, 24) (the c block): rows 0..26  at (c - 12) * 27 + r
-//   column c < 12 (the z block):        rows 24..26 at 324 + 3 c + (r - 24)
-constexpr int kGramZ = 12 * 27;
+//   column c in [12, 24) (the c block): rows 0..26  at (c - 12) * 28 + r
+//   column c < 12 (the z block):        rows 24..26 at 336 + 3 c + (r - 24)
+// The c-block pitch of 28 keeps every group of 4 rows 16-byte aligned.
+constexpr int kGramPitch = 28;
+constexpr int kGramZ = 12 * kGramPitch;
 constexpr int kGramSize = kGramZ + 12 * 3;
 
 template <int D>
@@ -214,7 +216,7 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
   // rows 0..D-1: the segment's normals (component-major); rows D..D+3: the
   // constant rows 1, k, k(k-1)/2 and a zero row, written once
   __shared__ __align__(16) float zbuf[Sh::ZROWS][ZP];
-  __shared__ double gram[kGramSize];  // FP64 Gram (compact), flushed into every segment
+  __shared__ __align__(16) double gram[kGramSize];  // FP64 Gram (compact), flushed into every segment
   __shared__ double eps_part[EPS_P][D];
   __shared__ double seg_eps[D];
   __shared__ double seg_sig[D];
@@ -250,17 +252,14 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
   }
   for (int e = l; e < kGramSize; e += 64) gram[e] = 0.0;
 
-  // accumulator v of lane l is Gram element [8(v/4) + 4h + v%4][l%32]: this
-  // lane's compact base and the entries it keeps
-  auto needed = [&](int r) -> bool {
-    const bool rz = r < D, rc = r >= kRowC && r < kRowC + D, rd = r >= kRowDet && r < kRowDet + 3;
-    return (is_c && (rz || rc || rd)) || (is_z && rd);
-  };
-  const int gbase = (is_c ? (row - kRowC) * 27 : kGramZ + 3 * row - kRowDet) + 4 * h;
-  uint32_t gmask = 0;
-#pragma unroll
-  for (int v = 0; v < 16; ++v)
-    if (needed(8 * (v / 4) + 4 * h + (v % 4))) gmask |= 1u << v;
+  // accumulator v of lane l is Gram element [8(v/4) + 4h + v%4][l%32].  The
+  // output needs, on a c column, rows 0..26 (v = 0..11 on both halves, v =
+  // 12..14 on the lower half: rows 24..26) and, on a z column, rows 24..26
+  // (v = 12..14, lower half).  Rows of unused components (dim < 12) hold
+  // zeros and land in their own, unread slots of the column.
+  double* const gcol = gram + (is_c ? (row - kRowC) * kGramPitch + 4 * h : 0);  // c lanes: v = 0..11
+  double* const gdet = is_c ? gram + (row - kRowC) * kGramPitch + kRowDet : gram + kGramZ + 3 * row;
+  const bool det_lane = h == 0 && (is_c || is_z);
 
   // (i, j) grid entries of this lane: e = l + 64 g
   int gi[NG], gj[NG];
@@ -402,9 +401,26 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
       add[12] = P0;
       add[13] = (a * P0 + P1) * dt;
       add[14] = ((a * a) * P0 + (2.0 * a + 1.0) * P1 + 2.0 * P2) * (dt * dt);
+      // two uniform regions instead of a branch per entry: the reads of a
+      // region issue together and the groups of 4 rows move as 16-byte pairs
+      if (is_c) {
 #pragma unroll
-      for (int v = 0; v < 16; ++v)
-        if (gmask & (1u << v)) gram[gbase + 8 * (v / 4) + (v % 4)] += add[v];
+        for (int q = 0; q < 3; ++q) {
+          double2* p = reinterpret_cast<double2*>(gcol + 8 * q);
+          double2 lo = p[0], hi2 = p[1];
+          lo.x += add[4 * q];
+          lo.y += add[4 * q + 1];
+          hi2.x += add[4 * q + 2];
+          hi2.y += add[4 * q + 3];
+          p[0] = lo;
+          p[1] = hi2;
+        }
+      }
+      if (det_lane) {
+        gdet[0] += add[12];
+        gdet[1] += add[13];
+        gdet[2] += add[14];
+      }
     }
     // FP64 per-segment terms on the (i, j) grid
 #pragma unroll
@@ -442,12 +458,12 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
     if (e < D * D) {
       const int i = gi[g], j = gj[g];
       const int lo = i < j ? i : j, hi = i < j ? j : i;  // one triangle: exactly symmetric
-      acc[(size_t)e * ld + walk] = bbA[g] + s2 * gram[hi * 27 + kRowC + lo];
-      acc[(size_t)(D * D + e) * ld + walk] = ebA[g] + s2 * gram[j * 27 + i];
+      acc[(size_t)e * ld + walk] = bbA[g] + s2 * gram[hi * kGramPitch + kRowC + lo];
+      acc[(size_t)(D * D + e) * ld + walk] = ebA[g] + s2 * gram[j * kGramPitch + i];
     }
   }
   if (l < 3 * D) {
-    acc[(size_t)(2 * D * D + mp * D + mi) * ld + walk] = mbA + s * gram[mi * 27 + kRowDet + mp];
+    acc[(size_t)(2 * D * D + mp * D + mi) * ld + walk] = mbA + s * gram[mi * kGramPitch + kRowDet + mp];
     acc[(size_t)(2 * D * D + (3 + mp) * D + mi) * ld + walk] = s * gram[kGramZ + 3 * mi + mp];
   }
 }
