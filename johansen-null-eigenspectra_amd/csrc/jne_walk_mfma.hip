@@ -57,6 +57,7 @@ namespace jne {
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef __fp16 fp16x2 __attribute__((ext_vector_type(2)));
 
 constexpr int kSegSteps = 64;  // steps per FP64 segment and per RNG round
 constexpr int kChunks = 4;     // 16-step MFMA chunks per segment
@@ -98,17 +99,28 @@ struct ChunkMfma<JNE_PREC_MFMA_FP16X2> {
   static constexpr bool kInterleaved = false;
   __device__ __forceinline__ static void run(const float (&x)[8], floatx16& acc, floatx16& acc2) {
     half8 hi, lo;
+#if JNE_MFMA_SPLIT_ROUND
+    // hi = x rounded to fp16 (packed pairs); x - hi is exact in fp32, and
+    // v_dot2_f32_f16 forms it from the packed pair directly: (hi0, hi1).(-1, 0) + x0
+    const fp16x2 m0 = {(__fp16)-1.0f, (__fp16)0.0f}, m1 = {(__fp16)0.0f, (__fp16)-1.0f};
+#pragma unroll
+    for (int k = 0; k < 8; k += 2) {
+      fp16x2 hp;
+      hp[0] = (__fp16)x[k];
+      hp[1] = (__fp16)x[k + 1];
+      hi[k] = (_Float16)hp[0];
+      hi[k + 1] = (_Float16)hp[1];
+      lo[k] = (_Float16)__builtin_amdgcn_fdot2(hp, m0, x[k], false);
+      lo[k + 1] = (_Float16)__builtin_amdgcn_fdot2(hp, m1, x[k + 1], false);
+    }
+#else
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-#if JNE_MFMA_SPLIT_ROUND
-      hi[k] = (_Float16)x[k];
-      lo[k] = (_Float16)(x[k] - (float)hi[k]);
-#else
       const float h = trunc_bits(x[k], 0xFFFFE000u);  // 11 significant bits
       hi[k] = (_Float16)h;
       lo[k] = (_Float16)(x[k] - h);
-#endif
     }
+#endif
     acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(hi, hi, acc, 0, 0, 0);
     acc2 = __builtin_amdgcn_mfma_f32_32x32x16_f16(hi, lo, acc2, 0, 0, 0);
     acc2 = __builtin_amdgcn_mfma_f32_32x32x16_f16(lo, hi, acc2, 0, 0, 0);
