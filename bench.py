@@ -13,9 +13,11 @@ state, model algebra and eigen solve).  The plain FP64 path is timed on the
 same batch in the same run, and the two outputs are compared walk by walk.
 
 Prints one JSON line with walks/s, the roofline figure of the dominant kernel
-(timed with HIP events on the stream it runs on) and the CPU baseline (the C
-restatement of the reference walk in oracle/, on a bounded sample of the same
-workload, on this host's cores).
+(timed with HIP events on the stream it runs on), the rate of the random stream
+alone at the same shape (the normals every walk path must draw: the ceiling the
+moment kernel is measured against in roofline.frac_of_stream) and the CPU
+baseline (the C restatement of the reference walk in oracle/, on a bounded
+sample of the same workload, on this host's cores).
 """
 from __future__ import annotations
 
@@ -189,6 +191,25 @@ def time_path(pkg, eng, prec_name, model, dim, T, seeds, out, n, steps, warmup, 
     return steps * n / (t1 - t0), (t1 - t0) / steps, mom, eig[0]
 
 
+def stream_rate(pkg, eng, dim, T, torch, stream, walks=16384, reps=5):
+    """Normals/s of the random stream alone (jne_walk_normals_device: Philox +
+    Box-Muller written to HBM, 4 B per normal) at the workload's shape: the
+    ceiling of any walk path that draws the same normals."""
+    dev = torch.device("cuda:0")
+    seeds = torch.arange(1, walks + 1, dtype=torch.int32, device=dev)
+    buf = torch.empty(walks * T * dim, dtype=torch.float32, device=dev)
+    eng.walk_normals_device(dim, T, seeds.data_ptr(), walks, buf.data_ptr(), hip_stream=stream)
+    torch.cuda.synchronize()
+    eng.profile_enable(True)
+    for _ in range(reps):
+        eng.walk_normals_device(dim, T, seeds.data_ptr(), walks, buf.data_ptr(), hip_stream=stream)
+    ms, launches, _ = eng.profile_read(pkg.KERNEL_NORMALS)
+    eng.profile_enable(False)
+    del buf
+    avg_ms = ms / max(launches, 1)
+    return walks * T * dim / (avg_ms * 1e-3), avg_ms
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -318,6 +339,11 @@ def main() -> int:
         "roofline": roofline,
         "kernels_ms": {"moments": mom_ms, "eigen": eig_ms, "launches": mom_launches},
     }
+
+    srate, sms = stream_rate(pkg, eng, dim, T, torch, stream)
+    roofline["stream_normals_per_s"] = srate
+    roofline["stream_write_GBps"] = srate * 4 / 1e9
+    roofline["frac_of_stream"] = roofline["normals_per_s"] / srate
 
     if args.compare:
         cwps, csps, cmom, ceig_ms = time_path(pkg, eng, args.compare, model, dim, T, seeds, out, n, args.steps,

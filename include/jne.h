@@ -109,6 +109,14 @@ int jne_eigenvalues_from_normals(jne_ctx *ctx, int model, int dim, uint32_t step
 int jne_walk_normals(jne_ctx *ctx, int dim, uint32_t steps, uint32_t rng_stream, const uint32_t *seeds, size_t n,
                      float *out);
 
+/* Same with device pointers (d_seeds: n seeds, d_out: n * steps * dim floats
+ * in device memory of ctx's device), enqueued on `hip_stream` (NULL = the
+ * context's stream) without synchronising; steps * dim < 2^32.  For callers
+ * that consume the stream on the device.  Also replaces gen_normal_matrix
+ * (src/rng_matrix.rs:11-37). */
+int jne_walk_normals_device(jne_ctx *ctx, int dim, uint32_t steps, uint32_t rng_stream, const uint32_t *d_seeds,
+                            size_t n, float *d_out, void *hip_stream);
+
 /* Block until all work enqueued on the context's stream has finished. */
 int jne_synchronize(jne_ctx *ctx);
 
@@ -120,6 +128,7 @@ int jne_synchronize(jne_ctx *ctx);
  * launches and the number of walks they processed. */
 #define JNE_KERNEL_MOMENTS 0 /* one-pass moment (Gram) accumulation */
 #define JNE_KERNEL_EIGEN 1   /* model algebra + eigen solve */
+#define JNE_KERNEL_NORMALS 2 /* the random stream alone (jne_walk_normals[_device]) */
 int jne_profile_enable(jne_ctx *ctx, int enable);
 int jne_profile_read(jne_ctx *ctx, int kernel, double *total_ms, uint64_t *launches, uint64_t *walks);
 

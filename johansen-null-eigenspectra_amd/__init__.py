@@ -58,6 +58,7 @@ MAX_DIM = 12
 
 KERNEL_MOMENTS = 0
 KERNEL_EIGEN = 1
+KERNEL_NORMALS = 2
 
 _lib = None
 _lib_lock = threading.Lock()
@@ -101,6 +102,9 @@ def load_library() -> ctypes.CDLL:
         L.jne_walk_normals.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32,
                                        ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
         L.jne_walk_normals.restype = ctypes.c_int
+        L.jne_walk_normals_device.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32,
+                                              ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p]
+        L.jne_walk_normals_device.restype = ctypes.c_int
         L.jne_synchronize.argtypes = [ctypes.c_void_p]
         L.jne_synchronize.restype = ctypes.c_int
         L.jne_profile_enable.argtypes = [ctypes.c_void_p, ctypes.c_int]
@@ -300,6 +304,14 @@ class Engine:
                                                len(s), out.ctypes.data))
         return out
 
+    def walk_normals_device(self, dim: int, steps: int, d_seeds: int, n: int, d_out: int, rng_stream: int = 0,
+                            hip_stream: int = 0):
+        """Device pointers (ints): d_out receives n*steps*dim floats; enqueues on hip_stream (0 = the
+        context's stream)."""
+        self._check(self._lib.jne_walk_normals_device(
+            self._ctx, int(dim), int(steps), int(rng_stream), ctypes.c_void_p(d_seeds), int(n),
+            ctypes.c_void_p(d_out), ctypes.c_void_p(hip_stream or None)))
+
     def synchronize(self):
         self._check(self._lib.jne_synchronize(self._ctx))
 
@@ -307,7 +319,7 @@ class Engine:
         self._check(self._lib.jne_profile_enable(self._ctx, 1 if enable else 0))
 
     def profile_read(self, kernel: int):
-        """(total_ms, launches, walks) of kernel KERNEL_MOMENTS or KERNEL_EIGEN."""
+        """(total_ms, launches, walks) of kernel KERNEL_MOMENTS, KERNEL_EIGEN or KERNEL_NORMALS."""
         ms = ctypes.c_double()
         nl = ctypes.c_uint64()
         nw = ctypes.c_uint64()

@@ -17,33 +17,46 @@
 
 namespace jne {
 
-// One thread per Philox call of one walk: writes normals 4q..4q+3.
+// Walk w0 + blockIdx.y; its Philox calls q = blockIdx.x*256 + tid, stepping by
+// the grid width, each writing normals 4q..4q+3 (16-byte stores when `vec`:
+// every walk's row of T*d floats starts 16-byte aligned).
 __global__ __launch_bounds__(256) void walk_normals_kernel(const uint32_t* __restrict__ seeds, uint32_t rng_stream,
-                                                           int d, uint32_t T, uint32_t n, uint64_t calls_per_walk,
+                                                           uint32_t w0, uint32_t total, uint32_t calls, bool vec,
                                                            float* __restrict__ out) {
-  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t walk = gid / calls_per_walk;
-  if (walk >= n) return;
-  const uint64_t q = gid - walk * calls_per_walk;
-  float v[4];
-  normals4(q, seeds[walk], rng_stream, v);
-  const uint64_t total = (uint64_t)T * d;
-  float* o = out + walk * total;
+  const uint32_t walk = w0 + blockIdx.y;
+  const uint32_t seed = seeds[walk];
+  float* const o = out + (size_t)walk * total;
+  for (uint32_t q = blockIdx.x * 256u + threadIdx.x; q < calls; q += gridDim.x * 256u) {
+    float v[4];
+    normals4(q, seed, rng_stream, v);
+    if (vec) {
+      *reinterpret_cast<float4*>(o + 4 * (size_t)q) = make_float4(v[0], v[1], v[2], v[3]);
+    } else {
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const uint64_t idx = 4 * q + k;
-    if (idx < total) o[idx] = v[k];
+      for (int k = 0; k < 4; ++k)
+        if (4 * q + k < total) o[4 * (size_t)q + k] = v[k];
+    }
   }
 }
 
 hipError_t launch_walk_normals(int d, const uint32_t* seeds, uint32_t rng_stream, uint32_t T, uint32_t n,
                                float* out, hipStream_t st) {
-  const uint64_t calls = ((uint64_t)T * d + 3) / 4;
-  const uint64_t threads = calls * n;
-  const uint64_t grid = (threads + 255) / 256;
-  hipLaunchKernelGGL(walk_normals_kernel, dim3((unsigned)grid), dim3(256), 0, st, seeds, rng_stream, d, T, n, calls,
-                     out);
-  return hipGetLastError();
+  // the caller keeps T * d < 2^32; a launch covers at most 65,535 walks (grid y)
+  const uint32_t total = T * (uint32_t)d;
+  const uint32_t calls = (total + 3) / 4;
+#ifndef JNE_NORMALS_GX
+#define JNE_NORMALS_GX 512u
+#endif
+  const uint32_t gx = (calls + 255) / 256 < JNE_NORMALS_GX ? (calls + 255) / 256 : JNE_NORMALS_GX;
+  const bool vec = (total & 3u) == 0u && (reinterpret_cast<uintptr_t>(out) & 15u) == 0u;
+  for (uint32_t w0 = 0; w0 < n; w0 += 65535u) {
+    const uint32_t gy = n - w0 < 65535u ? n - w0 : 65535u;
+    hipLaunchKernelGGL(walk_normals_kernel, dim3(gx, gy), dim3(256), 0, st, seeds, rng_stream, w0, total, calls, vec,
+                       out);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 }  // namespace jne
@@ -312,27 +325,54 @@ int jne_eigenvalues_from_normals(jne_ctx* ctx, int model, int dim, uint32_t step
   return finish_host_out(ctx, out, n * nf);
 }
 
+namespace {
+int check_normals_args(jne_ctx* ctx, int dim, uint32_t steps, size_t n) {
+  if (dim < 1 || dim > 255) return set_err(ctx, JNE_ERR_INVALID_ARG, "invalid dim %d", dim);
+  if (steps < 1 || steps > (1u << 30)) return set_err(ctx, JNE_ERR_INVALID_ARG, "invalid steps %u", steps);
+  if ((uint64_t)steps * dim > 0xFFFFFFFCull)
+    return set_err(ctx, JNE_ERR_INVALID_ARG, "jne_walk_normals: steps * dim must stay below 2^32");
+  if (n > 0xFFFFFFFFull) return set_err(ctx, JNE_ERR_INVALID_ARG, "jne_walk_normals: at most 2^32-1 walks");
+  return JNE_OK;
+}
+}  // namespace
+
 int jne_walk_normals(jne_ctx* ctx, int dim, uint32_t steps, uint32_t rng_stream, const uint32_t* seeds, size_t n,
                      float* out) {
   if (!ctx) return JNE_ERR_INVALID_ARG;
   std::lock_guard<std::mutex> lock(ctx->mu);
   ctx->err.clear();
-  if (dim < 1 || dim > 255) return set_err(ctx, JNE_ERR_INVALID_ARG, "invalid dim %d", dim);
-  if (steps < 1 || steps > (1u << 30)) return set_err(ctx, JNE_ERR_INVALID_ARG, "invalid steps %u", steps);
+  int rc = check_normals_args(ctx, dim, steps, n);
+  if (rc) return rc;
   if (n == 0) return JNE_OK;
   if (!seeds || !out) return set_err(ctx, JNE_ERR_INVALID_ARG, "seeds/out must not be NULL");
-  if (((uint64_t)steps * dim + 3) / 4 * n > (uint64_t)INT32_MAX * 256)
-    return set_err(ctx, JNE_ERR_INVALID_ARG, "jne_walk_normals: request too large");
   JNE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   const size_t total = n * (size_t)steps * dim;
-  int rc;
   if ((rc = ensure(ctx, &ctx->d_in, &ctx->in_bytes, n * sizeof(uint32_t)))) return rc;
   if ((rc = ensure(ctx, &ctx->d_out, &ctx->out_bytes, total * sizeof(float)))) return rc;
   JNE_HIP_CHECK(ctx, hipMemcpyAsync(ctx->d_in, seeds, n * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
-  JNE_HIP_CHECK(ctx, jne::launch_walk_normals(dim, (const uint32_t*)ctx->d_in, rng_stream, steps, (uint32_t)n,
-                                             (float*)ctx->d_out, ctx->stream));
+  JNE_HIP_CHECK(ctx, timed_launch(ctx, JNE_KERNEL_NORMALS, n, ctx->stream, [&] {
+                  return jne::launch_walk_normals(dim, (const uint32_t*)ctx->d_in, rng_stream, steps, (uint32_t)n,
+                                                  (float*)ctx->d_out, ctx->stream);
+                }));
   JNE_HIP_CHECK(ctx, hipMemcpyAsync(out, ctx->d_out, total * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
   JNE_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return JNE_OK;
+}
+
+int jne_walk_normals_device(jne_ctx* ctx, int dim, uint32_t steps, uint32_t rng_stream, const uint32_t* d_seeds,
+                            size_t n, float* d_out, void* hip_stream) {
+  if (!ctx) return JNE_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->err.clear();
+  int rc = check_normals_args(ctx, dim, steps, n);
+  if (rc) return rc;
+  if (n == 0) return JNE_OK;
+  if (!d_seeds || !d_out) return set_err(ctx, JNE_ERR_INVALID_ARG, "d_seeds/d_out must not be NULL");
+  JNE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = hip_stream ? (hipStream_t)hip_stream : ctx->stream;
+  JNE_HIP_CHECK(ctx, timed_launch(ctx, JNE_KERNEL_NORMALS, n, st, [&] {
+                  return jne::launch_walk_normals(dim, d_seeds, rng_stream, steps, (uint32_t)n, d_out, st);
+                }));
   return JNE_OK;
 }
 

@@ -27,6 +27,62 @@ def engine(jne):
     eng.close()
 
 
+class DeviceArray:
+    """A device allocation through the HIP runtime (ctypes), for the
+    device-pointer entry points: from_host / to_host copy synchronously."""
+
+    _hip = None
+
+    @classmethod
+    def hip(cls):
+        if cls._hip is None:
+            import ctypes
+
+            cls._hip = ctypes.CDLL("libamdhip64.so")
+        return cls._hip
+
+    def __init__(self, nbytes: int):
+        import ctypes
+
+        self.nbytes = int(nbytes)
+        self._p = ctypes.c_void_p()
+        rc = self.hip().hipMalloc(ctypes.byref(self._p), ctypes.c_size_t(max(self.nbytes, 1)))
+        assert rc == 0, f"hipMalloc({self.nbytes}) failed: {rc}"
+
+    @property
+    def ptr(self) -> int:
+        return self._p.value
+
+    def from_host(self, a, offset: int = 0):
+        import ctypes
+
+        import numpy as np
+
+        a = np.ascontiguousarray(a)
+        assert offset + a.nbytes <= self.nbytes
+        rc = self.hip().hipMemcpy(ctypes.c_void_p(self.ptr + offset), ctypes.c_void_p(a.ctypes.data),
+                                  ctypes.c_size_t(a.nbytes), 1)
+        assert rc == 0, rc
+
+    def to_host(self, shape, dtype, offset: int = 0):
+        import ctypes
+
+        import numpy as np
+
+        out = np.empty(shape, dtype=dtype)
+        assert offset + out.nbytes <= self.nbytes
+        assert self.hip().hipDeviceSynchronize() == 0
+        rc = self.hip().hipMemcpy(ctypes.c_void_p(out.ctypes.data), ctypes.c_void_p(self.ptr + offset),
+                                  ctypes.c_size_t(out.nbytes), 2)
+        assert rc == 0, rc
+        return out
+
+    def free(self):
+        if self._p.value:
+            self.hip().hipFree(self._p)
+            self._p.value = None
+
+
 @pytest.fixture(scope="session")
 def O():
     import oracle

@@ -30,6 +30,53 @@ def test_normals_bit_exact(engine, O):
     assert not np.array_equal(got[0], O.walk_normals(7, 0, 4, 50))
 
 
+def test_normals_device_entry_point(engine, O):
+    """jne_walk_normals_device equals the host entry point and the CPU stream:
+    more walks than one launch covers (65,535), a row of T*d floats that is
+    not a multiple of 4 (scalar stores), a walk longer than one pass of the
+    grid (strided calls), and an output pointer that is not 16-byte aligned."""
+    from conftest import DeviceArray
+
+    for dim, steps, n, offset in [(3, 5, 70000, 0), (12, 50000, 2, 0), (4, 25, 300, 4), (12, 100, 64, 0)]:
+        seeds = np.arange(1, n + 1, dtype=np.uint32)
+        seeds[-1] = 0xFFFFFFFF
+        d_seeds = DeviceArray(seeds.nbytes)
+        d_out = DeviceArray(n * steps * dim * 4 + offset)
+        try:
+            d_seeds.from_host(seeds)
+            engine.walk_normals_device(dim, steps, d_seeds.ptr, n, d_out.ptr + offset)
+            got = d_out.to_host((n, steps, dim), np.float32, offset)
+        finally:
+            d_seeds.free()
+            d_out.free()
+        host = engine.walk_normals(dim, steps, seeds)
+        assert np.array_equal(got.view(np.uint32), host.view(np.uint32)), (dim, steps, n, offset)
+        for k in [0, n // 2, n - 1]:
+            ref = O.walk_normals(int(seeds[k]), 0, dim, steps)
+            assert np.array_equal(got[k].view(np.uint32), ref.view(np.uint32)), (dim, steps, k)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mfma_fp16x2"])
+def test_eigenvalues_device_entry_point(engine, jne, prec):
+    """jne_eigenvalues_by_seed_device equals the host entry point bit for bit,
+    across the 2^18-walk chunks of one call."""
+    from conftest import DeviceArray
+
+    model, dim, steps, n = 4, 3, 70, (1 << 18) + 1000
+    seeds = np.arange(7, 7 + n, dtype=np.uint32)
+    d_seeds = DeviceArray(seeds.nbytes)
+    d_out = DeviceArray(n * dim * 8)
+    p = jne.PRECISIONS[prec]
+    try:
+        d_seeds.from_host(seeds)
+        engine.eigenvalues_by_seed_device(model, dim, steps, d_seeds.ptr, n, d_out.ptr, precision=p)
+        got = d_out.to_host((n, dim), np.float64)
+    finally:
+        d_seeds.free()
+        d_out.free()
+    assert np.array_equal(got, engine.eigenvalues_by_seed(model, dim, steps, seeds, precision=p))
+
+
 @pytest.mark.parametrize("model", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("dim", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12])
 def test_by_seed_matches_oracle_small(engine, O, model, dim):
