@@ -11,9 +11,10 @@
 // Outputs (w0,w1,w2,w3) of one call give normals 4q..4q+3 as
 // normal_pair(w0,w1) and normal_pair(w2,w3).  normal_pair is Box-Muller with
 // the logarithm and the sine/cosine replaced by fixed float32 polynomials
-// (coefficients from derive_rng_coeffs.py) so that only correctly rounded
-// IEEE operations are used; the stream is therefore bit-identical on any
-// IEEE-754 host that performs the same operation sequence.
+// (coefficients from derive_rng_coeffs.py) and a correctly rounded square
+// root, so that every result is that of correctly rounded IEEE operations;
+// the stream is therefore bit-identical on any IEEE-754 host that performs
+// the same operation sequence.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -51,17 +52,19 @@ __device__ __forceinline__ u32x4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_
 }
 
 // ln(u) for u in [2^-33, 1]: u = m * 2^e with m in [sqrt(1/2), sqrt(2)),
-// ln(m) = f + f^2 Q(f), f = m - 1 (exact).
+// ln(m) = f + f^2 Q(f), f = m - 1 (exact).  The split comes from
+// v_frexp_mant/v_frexp_exp (mant in [1/2, 1)): m = mant, or 2 mant below
+// sqrt(1/2) with e - 1 -- the same (m, e) as taking the exponent field.
 __device__ __forceinline__ float ln_poly(float u) {
 #pragma clang fp contract(off)
-  const uint32_t bits = __float_as_uint(u);
-  int e = (int)(bits >> 23) - 127;
-  uint32_t mbits = (bits & 0x007FFFFFu) | 0x3F800000u;
-  if (mbits > 0x3FB504F3u) {
-    mbits -= 0x00800000u;
-    e += 1;
-  }
-  const float f = __uint_as_float(mbits) - 1.0f;
+  const float mant = __builtin_amdgcn_frexp_mantf(u);
+  const int ex = __builtin_amdgcn_frexp_expf(u);
+  // 0x1.6a09e6p-1 is sqrt(1/2) rounded down: m = 2 mant whenever 2 mant is at
+  // most sqrt(2) rounded down, as with the exponent-field split
+  const bool lo = mant <= 0x1.6a09e6p-1f;
+  const float m = lo ? mant + mant : mant;
+  const int e = lo ? ex - 1 : ex;
+  const float f = m - 1.0f;
   float q = 0x1.15425cp-4f;
   q = __builtin_fmaf(q, f, -0x1.d962f2p-4f);
   q = __builtin_fmaf(q, f, 0x1.e3d33ap-4f);
@@ -78,11 +81,28 @@ __device__ __forceinline__ float ln_poly(float u) {
   return __builtin_fmaf(ef, 0x1.62e43p-1f, __builtin_fmaf(ef, -0x1.05c61p-29f, lnm));
 }
 
+// Correctly rounded sqrt(y) for y = +-0 or y in [2^-96, 2^126]: v_sqrt_f32
+// (within 1 ulp) and one correction step to the neighbour whose residual
+// has the right sign.  A general sqrtf adds scaling for tiny inputs and a
+// special-value check, neither needed on this range.
+__device__ __forceinline__ float sqrt_cr(float y) {
+#pragma clang fp contract(off)
+  const float s = __builtin_amdgcn_sqrtf(y);
+  const float dn = __uint_as_float(__float_as_uint(s) - 1u);
+  const float up = __uint_as_float(__float_as_uint(s) + 1u);
+  const float rdn = __builtin_fmaf(-dn, s, y);
+  const float rup = __builtin_fmaf(-up, s, y);
+  float r = rdn <= 0.0f ? dn : s;
+  r = rup > 0.0f ? up : r;
+  return r;
+}
+
 // Two Philox words -> two N(0,1) floats.
 __device__ __forceinline__ void normal_pair(uint32_t wa, uint32_t wb, float& z0, float& z1) {
 #pragma clang fp contract(off)
   const float u1 = __builtin_fmaf((float)wa, 0x1p-32f, 0x1p-33f);  // (0, 1]
-  const float r = __builtin_sqrtf(-2.0f * ln_poly(u1));
+  // -2 ln(u1) is +-0 or in [5.9e-8, 46.1]
+  const float r = sqrt_cr(-2.0f * ln_poly(u1));
   const uint32_t q = wb >> 30;
   const int32_t gi = (int32_t)(wb & 0x3FFFFFFFu) - (int32_t)0x20000000;
   const float g = (float)gi * 0x1p-30f;  // [-1/2, 1/2)
