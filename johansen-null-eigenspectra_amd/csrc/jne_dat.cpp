@@ -462,6 +462,7 @@ int jne_dat_writer_open(const char* path, int model, int dim, uint32_t steps, ui
       delete w;
       return fail(JNE_ERR_IO, os_error(errno));
     }
+    setvbuf(f, nullptr, _IOFBF, 2 << 20);  // before any other operation on the stream
     fseeko(f, 0, SEEK_END);
     const uint64_t flen = (uint64_t)ftello(f);
     if (flen > keep) {
@@ -486,7 +487,6 @@ int jne_dat_writer_open(const char* path, int model, int dim, uint32_t steps, ui
       }
     }
     fseeko(f, 0, SEEK_END);
-    setvbuf(f, nullptr, _IOFBF, 2 << 20);
     w->f = f;
   }
   if (rc) {
@@ -512,7 +512,8 @@ int jne_dat_writer_set_progress(jne_dat_writer* w, uint64_t total_runs) {
 int jne_dat_writer_append(jne_dat_writer* w, const uint32_t* seeds, const double* eigs, size_t n, int nf) {
   if (!w || (n && (!seeds || !eigs))) return fail(JNE_ERR_INVALID_ARG, "jne_dat_writer_append: NULL argument");
   if (n == 0) return 0;
-  if (nf < 0 || nf > 255)
+  if (nf < 1) return fail(JNE_ERR_FORMAT, "Invalid eigenvalue count: cannot be zero");
+  if (nf > 255)
     return fail(JNE_ERR_FORMAT, fmt("Too many eigenvalues: %s exceeds maximum of 255", commas((uint64_t)nf).c_str()));
   if (w->nf < 0) w->nf = nf;
   if (nf != w->nf)

@@ -94,6 +94,8 @@ struct Request {
   // device statistics: trace[runs], maxeig[runs] at arena offset
   size_t arena_off = 0;
   size_t bytes = 0;
+  bool has_arena = false;         // arena_off holds an allocation
+  int inflight = 0;               // chunks issued and not yet retired
   uint64_t issued = 0, done = 0;  // walks
   int status = 0;                 // JNE_OK or an error code
   std::string error;
@@ -159,12 +161,25 @@ struct jne_server {
 
 namespace {
 
+// A failed query keeps its statistics memory until none of its chunks is in
+// flight (they still write there); then the memory goes back to the arena.
+void release_if_drained(jne_server* s, const std::shared_ptr<Request>& r) {
+  if (r->status != 0 && r->has_arena && r->inflight == 0) {
+    s->arena.free(r->arena_off);
+    r->has_arena = false;
+    ++s->epoch;
+  }
+}
+
 void fail_request(jne_server* s, const std::shared_ptr<Request>& r, int code, const std::string& msg) {
-  r->status = code;
-  r->error = msg;
-  r->complete = true;
-  r->finished = Clock::now();
-  s->done_cv.notify_all();
+  if (r->status == 0) {
+    r->status = code;
+    r->error = msg;
+    r->complete = true;
+    r->finished = Clock::now();
+    s->done_cv.notify_all();
+  }
+  release_if_drained(s, r);
 }
 
 // Issues the next chunk of request r on lane L (called with s->mu held).
@@ -185,6 +200,7 @@ int issue(jne_server* s, Lane& L, const std::shared_ptr<Request>& r) {
   hipLaunchKernelGGL(walk_statistics, dim3((n + 255) / 256), dim3(256), 0, L.st, L.d_eigs, r->nf, n, trace, maxeig);
   if ((e = hipGetLastError()) != hipSuccess || (e = hipEventRecord(L.ev, L.st)) != hipSuccess) return JNE_ERR_HIP;
   r->issued += n;
+  ++r->inflight;
   L.req = r;
   L.walks = n;
   return 0;
@@ -207,6 +223,7 @@ void scheduler_loop(jne_server* s) {
       }
       if (!s->arena.alloc(r->bytes, &off)) break;  // wait for memory, keep arrival order
       r->arena_off = off;
+      r->has_arena = true;
       s->pending.pop_front();
       s->active.push_back(r);
     }
@@ -221,6 +238,7 @@ void scheduler_loop(jne_server* s) {
       }
       auto r = L.req;
       L.req.reset();
+      --r->inflight;
       if (q != hipSuccess) {
         fail_request(s, r, JNE_ERR_HIP, std::string("chunk failed: ") + hipGetErrorString(q));
         continue;
@@ -231,6 +249,7 @@ void scheduler_loop(jne_server* s) {
         s->to_finish.push_back(r);
         s->done_cv.notify_all();
       }
+      release_if_drained(s, r);  // a query failed by another lane's chunk
     }
     // hand out chunks round-robin over the admitted queries
     for (Lane& L : s->lanes) {
@@ -304,6 +323,7 @@ void finisher_loop(jne_server* s) {
     }
     lk.lock();
     s->arena.free(r->arena_off);
+    r->has_arena = false;
     ++s->epoch;
     if (code) {
       fail_request(s, r, code, msg);

@@ -149,6 +149,24 @@ def test_append_validation(jne, tmp_path):
         w.append([3], np.zeros((1, 256)))
     assert "Too many eigenvalues: 256 exceeds maximum of 255" in str(e.value)
     w.finish()
+    # a record without eigenvalues would not read back (the reader rejects a zero count)
+    w = jne.DatWriter(tmp_path / "z.dat", 4, 3, 10)
+    with pytest.raises(jne.JneError) as e:
+        w.append([1], np.zeros((1, 0)))
+    assert "Invalid eigenvalue count: cannot be zero" in str(e.value)
+    w.abandon()
+
+
+def test_255_eigenvalues_round_trip(jne, O, tmp_path):
+    """append_writer_test.rs: 255 eigenvalues per record is the largest count."""
+    path = tmp_path / "b.dat"
+    e = np.arange(2 * 255, dtype=np.float64).reshape(2, 255) / 7.0
+    w = jne.DatWriter(path, 2, 255, 11)
+    w.append([1, 300], e)
+    w.finish()
+    seeds, got = jne.read_dat(path)[:2]
+    assert seeds.tolist() == [1, 300] and np.array_equal(got, e)
+    assert path.read_bytes() == O.dat_bytes(2, 255, 11, [(1, e[0].tolist()), (300, e[1].tolist())])
 
 
 def test_dim_stored_as_u8(jne, O, tmp_path):

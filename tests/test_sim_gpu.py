@@ -68,6 +68,40 @@ def test_interrupted_file_resumes_cleanly(jne, engine, O, tmp_path):
         assert np.array_equal(np.array(e), full[sd - 1])
 
 
+def test_resume_fills_holes(jne, engine, tmp_path):
+    """integration/resumable.rs:4-76: seeds 2 and 4 removed from a finished
+    file are recomputed on the next run; every value is unchanged (the
+    reference compares eigenvalue sums within 1e-10; here they are identical)."""
+    s = sim(jne, engine, tmp_path, 3, 4, 103, 10)
+    s.run_simulation_quiet()
+    before = {sd: np.array(e) for sd, e in s.read_data()}
+    path = s.get_filename()
+    os.remove(path)
+    keep = np.array([sd for sd in sorted(before) if sd not in (2, 4)], dtype=np.uint32)
+    w = jne.DatWriter(path, 3, 4, 103)
+    w.append(keep, np.stack([before[int(sd)] for sd in keep]))
+    w.finish()
+    rem, completed = jne.remaining_seeds(path, 3, 4, 103, 10)
+    assert rem.tolist() == [2, 4] and completed == 8
+    s.run_simulation_quiet()
+    after = {sd: np.array(e) for sd, e in s.read_data()}
+    assert sorted(after) == list(range(1, 11))
+    for sd in range(1, 11):
+        assert np.array_equal(after[sd], before[sd]), sd
+
+
+def test_single_run_and_every_model(jne, engine, tmp_path):
+    """integration/edge_cases.rs (runs = 1 computes seed 1) and
+    multiple_models.rs (every model gives finite eigenvalues, d or d+1 of them)."""
+    for model in range(5):
+        s = sim(jne, engine, tmp_path, model, 3, 54, 1)
+        s.run_simulation_quiet()
+        data = s.read_data()
+        assert [r[0] for r in data] == [1]
+        e = np.array(data[0][1])
+        assert len(e) == jne.num_eigenvalues(model, 3) and np.isfinite(e).all()
+
+
 def test_incompatible_file_is_replaced(jne, engine, tmp_path, capfd):
     s = sim(jne, engine, tmp_path, 0, 2, 90, 50)
     w = jne.DatWriter(s.get_filename(), 0, 2, 91)  # same name, other steps (a hand-made clash)
