@@ -231,14 +231,16 @@ int jne_run_simulation(jne_ctx *ctx, int model, int dim, uint32_t steps, uint64_
  * precision): the same alone and in any mix of concurrent queries.
  * Queries are admitted in arrival order when their device footprint
  * (16 bytes per walk) fits the HBM budget; admitted queries share `lanes`
- * launch streams chunk by chunk (continuous batching).
+ * launch streams chunk by chunk (continuous batching).  The quantiles are
+ * selected on the device (radix sort of the per-walk statistics); only the
+ * six values come back to the host.
  * ------------------------------------------------------------------------ */
 #define JNE_ERR_TIMEOUT (-8)
 typedef struct jne_server jne_server;
 int jne_server_create(int device, uint64_t hbm_budget_bytes, int lanes, jne_server **out);
 /* Stops the scheduler; queries not yet waited for are dropped. */
 void jne_server_destroy(jne_server *server);
-/* Enqueue a query; returns at once with its id. */
+/* Enqueue a query (runs 1..2^31-1); returns at once with its id. */
 int jne_server_submit(jne_server *server, int model, int dim, uint32_t steps, uint64_t runs, uint32_t seed,
                       int precision, uint64_t *id);
 /* Wait for query id (timeout_ms < 0: no limit; JNE_ERR_TIMEOUT when it
