@@ -603,8 +603,8 @@ int jne_dat_read(const char* path, uint32_t* seeds, double* eigs, uint64_t capac
   *n_read = n;
   if (n > capacity) return fail(JNE_ERR_INVALID_ARG, fmt("jne_dat_read: %llu records exceed capacity %llu",
                                                           (unsigned long long)n, (unsigned long long)capacity));
-  if (seeds) memcpy(seeds, c.seeds.data(), n * sizeof(uint32_t));
-  if (eigs) memcpy(eigs, c.eigs.data(), c.eigs.size() * sizeof(double));
+  if (seeds && n) memcpy(seeds, c.seeds.data(), n * sizeof(uint32_t));
+  if (eigs && !c.eigs.empty()) memcpy(eigs, c.eigs.data(), c.eigs.size() * sizeof(double));
   return 0;
 }
 
@@ -662,7 +662,9 @@ int jne_dat_statistics(const char* path, uint64_t runs, int statistic, double* o
   uint64_t k = 0;
   for (size_t i = 0; i < c.seeds.size(); ++i) {
     if (runs && c.seeds[i] > runs) continue;  // read_data keeps seed <= num_runs (simulation.rs:62-67)
-    const double* e = &c.eigs[i * (size_t)c.nf];
+    // a scanned file may hold zero-count records (reader.rs:180-191): then the
+    // sum is 0 and the maximum stays at the fold's start
+    const double* e = c.eigs.data() + i * (size_t)c.nf;
     double v;
     if (statistic == JNE_STAT_TRACE) {  // simulation_analyzers.rs:26-31
       v = 0.0;
