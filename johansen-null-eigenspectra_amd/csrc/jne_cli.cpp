@@ -229,6 +229,20 @@ int parse(int argc, char** argv, CliArgs* a) {
     fprintf(stderr, "Error: steps, runs, and dimension parameters must be greater than 0\n");
     return 1;
   }
+  // the engine's limits (the reference takes any usize; its file stores steps as u32)
+  if (a->dim_end > JNE_MAX_DIM) {
+    fprintf(stderr, "Error: dimension %llu is above the supported maximum of %d\n", (unsigned long long)a->dim_end,
+            JNE_MAX_DIM);
+    return 1;
+  }
+  if (a->steps < 2 || a->steps > (1ull << 30)) {
+    fprintf(stderr, "Error: steps must be between 2 and %s\n", commas(1ull << 30).c_str());
+    return 1;
+  }
+  if (a->runs > 0xFFFFFFFFull) {
+    fprintf(stderr, "Error: runs must be at most %s (seeds are u32)\n", commas(0xFFFFFFFFull).c_str());
+    return 1;
+  }
   if (a->threads >= 0 && !validate_threads(a->threads)) return 1;
   return 0;
 }
