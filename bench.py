@@ -191,6 +191,29 @@ def time_path(pkg, eng, prec_name, model, dim, T, seeds, out, n, steps, warmup, 
     return steps * n / (t1 - t0), (t1 - t0) / steps, mom, eig[0]
 
 
+def unit_rates_for(key, walks_per_launch, avg_ms):
+    """Achieved TFLOP/s and fraction of peak of the matrix cores, the FP32 VALU
+    and the FP64 VALU: per-walk FLOP by unit from the committed counter runs
+    (profiles/unit_rates.json) over this run's kernel time."""
+    path = os.path.join(ROOT, "profiles", "unit_rates.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f).get(key)
+    if not rec:
+        return None
+    per_s = walks_per_launch / (avg_ms * 1e-3) / 1e12
+    out = {}
+    for unit, field, peak in [("mfma_f16", "mfma_f16_flop_per_walk", 2500.0),
+                              ("fp32_valu", "fp32_valu_flop_per_walk", 157.3),
+                              ("fp64_valu", "fp64_valu_flop_per_walk", 78.6)]:
+        out[unit] = {"tflops": rec[field] * per_s, "peak": peak, "frac": rec[field] * per_s / peak}
+    if rec.get("mfma_busy_cycles"):
+        out["mfma_coexec_frac"] = rec["mfma_coexec_cycles"] / rec["mfma_busy_cycles"]
+    out["source"] = "FLOP per walk from rocprofv3 counters (profiles/unit_rates.json) / HIP-event kernel time"
+    return out
+
+
 def stream_rate(pkg, eng, dim, T, torch, stream, walks=16384, reps=5):
     """Normals/s of the random stream alone (jne_walk_normals_device: Philox +
     Box-Muller written to HBM, 4 B per normal) at the workload's shape: the
@@ -295,6 +318,7 @@ def main() -> int:
         if rec:
             traffic = (rec["fetch_bytes_per_walk"] + rec["write_bytes_per_walk"]) * walks_per_launch
 
+    unit_rates = unit_rates_for(f"walk_moments_{args.precision}_m{model}_d{dim}_T{T}", walks_per_launch, avg_ms)
     roofline = {
         "bound": "mfma",
         "achieved": achieved,
@@ -309,6 +333,7 @@ def main() -> int:
         "avg_launch_ms": avg_ms,
         "walks_per_launch": walks_per_launch,
         "normals_per_s": wps * T * dim,
+        "unit_rates": unit_rates,
     }
     if args.precision in MFMA_PER_CHUNK:
         issued = MFMA_PER_CHUNK[args.precision] * 2 * 32 * 32 * 16 * ((T + 15) // 16)
@@ -350,11 +375,14 @@ def main() -> int:
                                               args.warmup, stream, torch)
         ref = out.view(n, nf)[:k].cpu().numpy()
         rel = relative_floor(head, ref)
+        c_avg = cmom[0] / max(cmom[1], 1)
         line["compare"] = {
             "precision": args.compare,
             "value": cwps,
             "ms_per_step": csps * 1e3,
-            "moments_avg_launch_ms": cmom[0] / max(cmom[1], 1),
+            "moments_avg_launch_ms": c_avg,
+            "unit_rates": unit_rates_for(f"walk_moments_{args.compare}_m{model}_d{dim}_T{T}",
+                                         cmom[2] / max(cmom[1], 1), c_avg),
             "speedup": wps / cwps,
             "max_rel_diff": float(rel.max()),
             "mean_rel_diff": float(rel.mean()),
