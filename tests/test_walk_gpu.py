@@ -96,6 +96,17 @@ def test_by_seed_matches_oracle_large(engine, O, model, dim, steps):
     assert_eigs_close(got, ref, RTOL, f"model {model} dim {dim} steps {steps}")
 
 
+@pytest.mark.parametrize("model,dim,steps,n", [(0, 2, 1000, 20000), (2, 5, 5000, 2000), (4, 12, 10000, 400)])
+def test_workload_shapes_many_walks(engine, O, model, dim, steps, n):
+    """The plain-CPU, plain-GPU and headline workload shapes on many walks
+    (seeds spread over the u32 range) against the C restatement of the
+    reference walk, at the FP64 tolerance."""
+    seeds = np.unique(np.linspace(1, 0xFFFFFFFF, n).astype(np.uint64)).astype(np.uint32)
+    got = engine.eigenvalues_by_seed(model, dim, steps, seeds)
+    ref = O.walks_by_seed_c(model, dim, steps, seeds, threads=8)
+    assert_eigs_close(got, ref, RTOL, f"model {model} dim {dim} steps {steps}")
+
+
 def test_from_normals_equals_by_seed(engine, O):
     model, dim, steps = 4, 6, 300
     seeds = np.arange(1, 40, dtype=np.uint32)
