@@ -286,15 +286,33 @@ struct SymReg {
   __device__ __forceinline__ double& operator()(int i, int j) { return v[idx(i, j)]; }
 };
 
+// 1/x and 1/sqrt(x) in FP64: the hardware estimate and one Newton step
+// (error a few ulp; a rotation needs orthogonality, not correct rounding)
+__device__ __forceinline__ double rcp_nr(double x) {
+  const double r = __builtin_amdgcn_rcp(x);
+  return fma(r, fma(-x, r, 1.0), r);
+}
+__device__ __forceinline__ double rsq_nr(double x) {
+  const double r = __builtin_amdgcn_rsq(x);
+  return r * fma(-0.5 * x * r, r, 1.5);
+}
+
 template <int N>
 __device__ __forceinline__ void jacobi_rotate(SymReg<N>& a, int p, int q) {
   const double apq = a(p, q);
-  if (apq == 0.0) return;
   const double app = a(p, p), aqq = a(q, q);
-  const double theta = (aqq - app) / (2.0 * apq);
-  double t = 1.0 / (fabs(theta) + sqrt(fma(theta, theta, 1.0)));
+  // threshold Jacobi: below 1e-18 sqrt(|app aqq|), zeroing a_pq moves the
+  // eigenvalues by at most ~1e-18 of themselves (a(p, q) = 0 is such a case)
+  if (apq * apq <= 1e-36 * fabs(app * aqq)) {
+    a(p, q) = 0.0;
+    return;
+  }
+  // theta clamped so theta^2 stays finite (t ~ 1/(2 theta) there anyway)
+  const double theta = fmin(fmax((aqq - app) * rcp_nr(2.0 * apq), -1e150), 1e150);
+  const double h = fma(theta, theta, 1.0);
+  double t = rcp_nr(fabs(theta) + h * rsq_nr(h));
   if (theta < 0.0) t = -t;
-  const double c = 1.0 / sqrt(fma(t, t, 1.0));
+  const double c = rsq_nr(fma(t, t, 1.0));
   const double s = t * c;
 #pragma unroll
   for (int k = 0; k < N; ++k) {
