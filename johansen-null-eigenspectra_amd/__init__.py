@@ -4,13 +4,27 @@ A thin ctypes layer over ``lib/libjne.so`` (the C ABI of ``include/jne.h``).
 The compute runs only in the HIP kernels of that library: if the library is
 missing or no GPU is visible, the calls raise -- there is no CPU fallback.
 
-Mirrors the reference's interface for the hot path:
+Mirrors the reference's interfaces around the walk path:
 
 * ``calculate_eigenvalues(dim, steps, seed, model)`` -- the per-walk operator
   of ``src/johansen_statistics.rs:59-85`` (returns |rho| sorted descending);
-* ``Engine.eigenvalues_by_seed`` -- the batched form used in place of the
-  per-seed rayon loop of ``src/data_storage/parallel_compute.rs:14-41``;
-* ``JohansenModel`` -- the model enum of ``src/johansen_models.rs:6-141``.
+* ``Engine`` -- the batched forms used in place of the per-seed rayon loop of
+  ``src/data_storage/parallel_compute.rs:14-41`` (host or device pointers, by
+  seed or from caller-supplied normals, and the device stream itself);
+* ``JohansenModel`` -- the model enum of ``src/johansen_models.rs:6-141``;
+* ``EigenvalueSimulation`` -- the library facade of
+  ``src/data_storage/simulation.rs`` (runs streamed to the ``.dat`` V6 file
+  with resume, ``read_data``/``read_all_data``, trace and max-eigenvalue
+  percentiles of ``src/simulation_analyzers.rs``);
+* ``DatWriter``, ``read_dat``, ``dat_info``, ``remaining_seeds``,
+  ``percentiles`` -- the ``.dat`` store of ``src/data_storage/``;
+* ``Server`` -- critical-value queries (no counterpart in the reference):
+  continuous batching on the device, quantiles selected on the device.
+
+Precisions (``PRECISIONS``): ``fp64`` (the reference-equivalent path),
+``mfma_fp16x2`` / ``mfma_bf16x3`` / ``mfma_fp32`` / ``mfma_bf16x2`` (moment
+Gram on the matrix cores), ``fp32_kahan`` (compensated FP32 VALU check) and
+``auto`` (the faster of fp64 and mfma_fp16x2 for each dim).
 """
 from __future__ import annotations
 
