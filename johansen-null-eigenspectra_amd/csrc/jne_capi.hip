@@ -69,6 +69,9 @@ struct jne_ctx {
   // device scratch: per-walk moments of one chunk
   double* d_acc = nullptr;
   size_t acc_bytes = 0;
+  // device scratch: M = W^T W of one chunk for the two-kernel eigen solve
+  double* d_m = nullptr;
+  size_t m_bytes = 0;
   // device staging for the host-pointer entry points
   void* d_in = nullptr;
   size_t in_bytes = 0;
@@ -191,6 +194,8 @@ int run_walks(jne_ctx* ctx, int model, int dim, uint32_t steps, uint32_t rng_str
   const size_t chunk = std::min(n, kChunkWalks);
   int rc = ensure(ctx, (void**)&ctx->d_acc, &ctx->acc_bytes, chunk * mc * sizeof(double));
   if (rc) return rc;
+  rc = ensure(ctx, (void**)&ctx->d_m, &ctx->m_bytes, chunk * (size_t)jne::eigen_m_count(dim) * sizeof(double));
+  if (rc) return rc;
   for (size_t off = 0; off < n; off += chunk) {
     const uint32_t cn = (uint32_t)std::min(chunk, n - off);
     const uint32_t* s = d_seeds ? d_seeds + off : nullptr;
@@ -203,7 +208,8 @@ int run_walks(jne_ctx* ctx, int model, int dim, uint32_t steps, uint32_t rng_str
                                                          cn, st);
                   }));
     JNE_HIP_CHECK(ctx, timed_launch(ctx, JNE_KERNEL_EIGEN, cn, st, [&] {
-                    return jne::launch_walk_eigen_fp64(model, dim, ctx->d_acc, cn, steps, cn, d_out + off * nf, st);
+                    return jne::launch_walk_eigen_fp64(model, dim, ctx->d_acc, cn, steps, cn, d_out + off * nf,
+                                                       ctx->d_m, st);
                   }));
   }
   return JNE_OK;
@@ -256,6 +262,7 @@ void jne_destroy(jne_ctx* ctx) {
   release_timed(ctx);
   for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
   if (ctx->d_acc) (void)hipFree(ctx->d_acc);
+  if (ctx->d_m) (void)hipFree(ctx->d_m);
   if (ctx->d_in) (void)hipFree(ctx->d_in);
   if (ctx->d_out) (void)hipFree(ctx->d_out);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

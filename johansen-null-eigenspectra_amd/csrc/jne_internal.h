@@ -92,8 +92,11 @@ hipError_t launch_walk_moments_fp64(int d, bool kahan, const uint32_t* seeds, co
 hipError_t launch_walk_moments_mfma(int d, int precision, const uint32_t* seeds, const double* zin,
                                     uint32_t rng_stream, uint32_t T, uint32_t n, double* acc, size_t ld,
                                     hipStream_t st);
+// mbuf: (eigen_m_count(d)) * ld doubles of scratch for the two-kernel form
+// (dims whose fused solve would hold fewer than 2 waves per SIMD)
+constexpr int eigen_m_count(int d) { return d * (d + 1) / 2 + 1; }
 hipError_t launch_walk_eigen_fp64(int model, int d, const double* acc, size_t ld, uint32_t T, uint32_t n,
-                                  double* out, hipStream_t st);
+                                  double* out, double* mbuf, hipStream_t st);
 hipError_t launch_walk_normals(int d, const uint32_t* seeds, uint32_t rng_stream, uint32_t T, uint32_t n,
                                float* out, hipStream_t st);
 

@@ -346,17 +346,24 @@ __device__ __forceinline__ void jacobi_eigvals(SymReg<N>& a) {
 }
 
 template <int MODEL, int D>
-__global__ __launch_bounds__(64) void walk_eigen_fp64(const double* __restrict__ acc, size_t ld, uint32_t T,
-                                                      uint32_t n_walks, double* __restrict__ out) {
-  constexpr int NX = (MODEL == 1 || MODEL == 3) ? D + 1 : D;  // nF
-  constexpr int NB = (MODEL == 2 || MODEL == 4) ? D - 1 : D;  // b rows in X
-  constexpr int NZ = MODEL == 4 ? 2 : ((MODEL == 2 || MODEL == 3) ? 1 : 0);
-  __shared__ double wsh[NX * D][64];
+struct EigenShape {
+  static constexpr int NX = (MODEL == 1 || MODEL == 3) ? D + 1 : D;  // nF
+  static constexpr int NB = (MODEL == 2 || MODEL == 4) ? D - 1 : D;  // b rows in X
+  static constexpr int NZ = MODEL == 4 ? 2 : ((MODEL == 2 || MODEL == 3) ? 1 : 0);
+  // one workgroup's W in LDS: NX*D doubles per walk.  Above 20 KiB fewer than
+  // 8 workgroups (2 waves per SIMD) fit a CU, and the solve runs as two
+  // kernels: the algebra up to M = W^T W, then Jacobi on M without LDS.
+  static constexpr bool kSplit = NX * D * 64 * 8 > 20 * 1024;
+};
 
-  const int tid = threadIdx.x;
-  const size_t w = (size_t)blockIdx.x * 64 + tid;
-  const bool valid = w < n_walks;
-  const size_t wc = valid ? w : (size_t)(n_walks - 1);
+// Model algebra, Cholesky of S11 and M = W^T W for walk wc (the prefix of the
+// solve; W goes through the workgroup's LDS buffer wsh).
+template <int MODEL, int D>
+__device__ __forceinline__ void eigen_prep(const double* __restrict__ acc, size_t ld, uint32_t T, size_t wc,
+                                           int tid, double (*wsh)[64], SymReg<D>& M, bool& pd_out) {
+  constexpr int NX = EigenShape<MODEL, D>::NX;
+  constexpr int NB = EigenShape<MODEL, D>::NB;
+  constexpr int NZ = EigenShape<MODEL, D>::NZ;
   auto A = [&](int e) -> double { return acc[(size_t)e * ld + wc]; };
   auto GBB = [&](int i, int j) { return A(i * D + j); };
   auto GEB = [&](int k, int j) { return A(D * D + k * D + j); };
@@ -480,7 +487,6 @@ __global__ __launch_bounds__(64) void walk_eigen_fp64(const double* __restrict__
   }
 
   // M = W^T W (D x D)
-  SymReg<D> M;
 #pragma unroll
   for (int k = 0; k < D * (D + 1) / 2; ++k) M.v[k] = 0.0;
 #pragma unroll 1
@@ -494,7 +500,13 @@ __global__ __launch_bounds__(64) void walk_eigen_fp64(const double* __restrict__
       for (int l = k; l < D; ++l) M(k, l) = fma(row[k], row[l], M(k, l));
     }
   }
+  pd_out = pd;
+}
 
+// Jacobi on M, |lambda| sorted descending, NX - D structural zeros; NaN rows
+// for a walk whose S11 was not positive definite.
+template <int D, int NX>
+__device__ __forceinline__ void eigen_finish(SymReg<D>& M, bool pd, double* __restrict__ o) {
   jacobi_eigvals<D>(M);
 
   double ev[D];
@@ -511,13 +523,62 @@ __global__ __launch_bounds__(64) void walk_eigen_fp64(const double* __restrict__
       ev[k + 1] = lo;
     }
   }
-  if (!valid) return;
-  double* o = out + w * NX;
   const double nan = __longlong_as_double(0x7FF8000000000000LL);
 #pragma unroll
   for (int k = 0; k < D; ++k) o[k] = pd ? ev[k] : nan;
   if constexpr (NX > D) o[D] = pd ? 0.0 : nan;
 }
+
+// One thread per walk: the whole solve in one kernel (small D).
+template <int MODEL, int D>
+__global__ __launch_bounds__(64) void walk_eigen_fp64(const double* __restrict__ acc, size_t ld, uint32_t T,
+                                                      uint32_t n_walks, double* __restrict__ out) {
+  constexpr int NX = EigenShape<MODEL, D>::NX;
+  __shared__ double wsh[NX * D][64];
+  const int tid = threadIdx.x;
+  const size_t w = (size_t)blockIdx.x * 64 + tid;
+  const bool valid = w < n_walks;
+  const size_t wc = valid ? w : (size_t)(n_walks - 1);
+  SymReg<D> M;
+  bool pd;
+  eigen_prep<MODEL, D>(acc, ld, T, wc, tid, wsh, M, pd);
+  if (!valid) return;
+  eigen_finish<D, NX>(M, pd, out + w * NX);
+}
+
+// Split form, first kernel: M and the definiteness flag, entry-major
+// (m[e * ld + walk], e < D(D+1)/2; e = D(D+1)/2 holds 1 or 0).
+template <int MODEL, int D>
+__global__ __launch_bounds__(64) void walk_eigen_prep(const double* __restrict__ acc, size_t ld, uint32_t T,
+                                                      uint32_t n_walks, double* __restrict__ m) {
+  constexpr int NX = EigenShape<MODEL, D>::NX;
+  __shared__ double wsh[NX * D][64];
+  const int tid = threadIdx.x;
+  const size_t w = (size_t)blockIdx.x * 64 + tid;
+  const bool valid = w < n_walks;
+  const size_t wc = valid ? w : (size_t)(n_walks - 1);
+  SymReg<D> M;
+  bool pd;
+  eigen_prep<MODEL, D>(acc, ld, T, wc, tid, wsh, M, pd);
+  if (!valid) return;
+#pragma unroll
+  for (int e = 0; e < D * (D + 1) / 2; ++e) m[(size_t)e * ld + w] = M.v[e];
+  m[(size_t)(D * (D + 1) / 2) * ld + w] = pd ? 1.0 : 0.0;
+}
+
+// Split form, second kernel: Jacobi from M in registers, no LDS.
+template <int D, int NX>
+__global__ __launch_bounds__(64) void walk_eigen_jacobi(const double* __restrict__ m, size_t ld, uint32_t n_walks,
+                                                        double* __restrict__ out) {
+  const size_t w = (size_t)blockIdx.x * 64 + threadIdx.x;
+  if (w >= n_walks) return;
+  SymReg<D> M;
+#pragma unroll
+  for (int e = 0; e < D * (D + 1) / 2; ++e) M.v[e] = m[(size_t)e * ld + w];
+  const bool pd = m[(size_t)(D * (D + 1) / 2) * ld + w] != 0.0;
+  eigen_finish<D, NX>(M, pd, out + w * NX);
+}
+
 
 // ---------------------------------------------------------------------------
 // launchers
@@ -544,10 +605,19 @@ static hipError_t launch_moments_d(bool kahan, const uint32_t* seeds, const doub
 }
 
 template <int MODEL, int D>
-static hipError_t launch_eigen_md(const double* acc, size_t ld, uint32_t T, uint32_t n, double* out,
+static hipError_t launch_eigen_md(const double* acc, size_t ld, uint32_t T, uint32_t n, double* out, double* mbuf,
                                   hipStream_t st) {
   const unsigned grid = (unsigned)((n + 63) / 64);
-  hipLaunchKernelGGL((walk_eigen_fp64<MODEL, D>), dim3(grid), dim3(64), 0, st, acc, ld, T, n, out);
+  if constexpr (EigenShape<MODEL, D>::kSplit) {
+    if (mbuf == nullptr) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((walk_eigen_prep<MODEL, D>), dim3(grid), dim3(64), 0, st, acc, ld, T, n, mbuf);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((walk_eigen_jacobi<D, EigenShape<MODEL, D>::NX>), dim3(grid), dim3(64), 0, st, mbuf, ld, n,
+                       out);
+  } else {
+    hipLaunchKernelGGL((walk_eigen_fp64<MODEL, D>), dim3(grid), dim3(64), 0, st, acc, ld, T, n, out);
+  }
   return hipGetLastError();
 }
 
@@ -567,11 +637,11 @@ hipError_t launch_walk_moments_fp64(int d, bool kahan, const uint32_t* seeds, co
 
 template <int MODEL>
 static hipError_t launch_eigen_m(int d, const double* acc, size_t ld, uint32_t T, uint32_t n, double* out,
-                                 hipStream_t st) {
+                                 double* mbuf, hipStream_t st) {
   switch (d) {
 #define JNE_CASE(DD) \
   case DD:           \
-    return launch_eigen_md<MODEL, DD>(acc, ld, T, n, out, st);
+    return launch_eigen_md<MODEL, DD>(acc, ld, T, n, out, mbuf, st);
     JNE_CASE(1) JNE_CASE(2) JNE_CASE(3) JNE_CASE(4) JNE_CASE(5) JNE_CASE(6)
     JNE_CASE(7) JNE_CASE(8) JNE_CASE(9) JNE_CASE(10) JNE_CASE(11) JNE_CASE(12)
 #undef JNE_CASE
@@ -581,13 +651,13 @@ static hipError_t launch_eigen_m(int d, const double* acc, size_t ld, uint32_t T
 }
 
 hipError_t launch_walk_eigen_fp64(int model, int d, const double* acc, size_t ld, uint32_t T, uint32_t n,
-                                  double* out, hipStream_t st) {
+                                  double* out, double* mbuf, hipStream_t st) {
   switch (model) {
-    case 0: return launch_eigen_m<0>(d, acc, ld, T, n, out, st);
-    case 1: return launch_eigen_m<1>(d, acc, ld, T, n, out, st);
-    case 2: return launch_eigen_m<2>(d, acc, ld, T, n, out, st);
-    case 3: return launch_eigen_m<3>(d, acc, ld, T, n, out, st);
-    case 4: return launch_eigen_m<4>(d, acc, ld, T, n, out, st);
+    case 0: return launch_eigen_m<0>(d, acc, ld, T, n, out, mbuf, st);
+    case 1: return launch_eigen_m<1>(d, acc, ld, T, n, out, mbuf, st);
+    case 2: return launch_eigen_m<2>(d, acc, ld, T, n, out, mbuf, st);
+    case 3: return launch_eigen_m<3>(d, acc, ld, T, n, out, mbuf, st);
+    case 4: return launch_eigen_m<4>(d, acc, ld, T, n, out, mbuf, st);
     default: return hipErrorInvalidValue;
   }
 }
