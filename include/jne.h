@@ -43,13 +43,13 @@ extern "C" {
 #define JNE_PREC_MFMA_BF16X2 3 /* MFMA, bf16 hi+lo operands, fp32 accumulate */
 #define JNE_PREC_MFMA_BF16X3 4 /* MFMA, bf16 hi+mid+lo operands, fp32 accumulate */
 #define JNE_PREC_MFMA_FP32 5   /* MFMA, fp32 operands, fp32 accumulate */
-#define JNE_PREC_AUTO 6        /* per dim, the faster of FP64 and MFMA_FP16X2 */
+#define JNE_PREC_AUTO 6        /* per (dim, steps), the faster of FP64 and MFMA_FP16X2 */
 /* The MFMA modes keep the state B and the per-64-step segment terms in FP64
  * and put only within-segment corrections on the matrix cores.  Both paths
  * draw the same normals, whose generation bounds them both; measured on
- * MI355X the MFMA pass is faster for dims 11 and 12 only, so JNE_PREC_AUTO
- * runs MFMA_FP16X2 there and FP64 below (FP64 results meet the mixed-path
- * tolerance trivially). */
+ * MI355X the MFMA pass is faster for dims 11-12 from 300 steps and dims 9-10
+ * from 1,000 steps, so JNE_PREC_AUTO runs MFMA_FP16X2 there and FP64
+ * elsewhere (FP64 results meet the mixed-path tolerance trivially). */
 
 #define JNE_MAX_DIM 12
 

@@ -24,7 +24,7 @@ Mirrors the reference's interfaces around the walk path:
 Precisions (``PRECISIONS``): ``fp64`` (the reference-equivalent path),
 ``mfma_fp16x2`` / ``mfma_bf16x3`` / ``mfma_fp32`` / ``mfma_bf16x2`` (moment
 Gram on the matrix cores), ``fp32_kahan`` (compensated FP32 VALU check) and
-``auto`` (the faster of fp64 and mfma_fp16x2 for each dim).
+``auto`` (the faster of fp64 and mfma_fp16x2 for each dim and length).
 """
 from __future__ import annotations
 

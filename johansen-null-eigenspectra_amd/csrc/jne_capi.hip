@@ -183,12 +183,18 @@ hipError_t timed_launch(jne_ctx* ctx, int kernel, uint64_t walks, hipStream_t st
 }
 
 // Enqueue the walk path for n walks (device pointers) on stream st.
-// JNE_PREC_AUTO: the MFMA pass from this dim up (measured crossover, model 2/4, T = 10^3..10^4)
-constexpr int kAutoMfmaMinDim = 11;
+// JNE_PREC_AUTO: the MFMA pass where it measured faster than the FP64 pass
+// (walks/s, model 4, T = 100..10^4): dims 11-12 from 300 steps, dims 9-10
+// from 1,000 steps; short walks carry the MFMA pass's per-walk setup.
+bool auto_uses_mfma(int dim, uint32_t steps) {
+  if (dim >= 11) return steps >= 300;
+  if (dim >= 9) return steps >= 1000;
+  return false;
+}
 
 int run_walks(jne_ctx* ctx, int model, int dim, uint32_t steps, uint32_t rng_stream, const uint32_t* d_seeds,
               const double* d_z, size_t n, int precision, double* d_out, hipStream_t st) {
-  if (precision == JNE_PREC_AUTO) precision = dim >= kAutoMfmaMinDim ? JNE_PREC_MFMA_FP16X2 : JNE_PREC_FP64;
+  if (precision == JNE_PREC_AUTO) precision = auto_uses_mfma(dim, steps) ? JNE_PREC_MFMA_FP16X2 : JNE_PREC_FP64;
   const int nf = jne::num_eigenvalues(model, dim);
   const size_t mc = (size_t)jne::moment_count(dim);
   const size_t chunk = std::min(n, kChunkWalks);
