@@ -8,9 +8,11 @@
 //   GeneralizedEigen (dggev)    src/johansen_statistics.rs:35-45
 // without materialising B, dB or F.  Kernel 1 streams the walk once and keeps
 // the moments of x_t = [b_t, 1, u_t, u_t^2, e_t] (b_t = B_t before step t,
-// e_t = B_{t+1} - B_t, u_t = (t+1)/T); kernel 2 forms S11 and S01 of the
-// requested model from those moments (demeaning and detrending are Schur
-// complements on the {1, u} columns) and solves the eigenproblem.
+// e_t = B_{t+1} - B_t, u_t = (t+1)/T) in FP64 (or, as the compensated check,
+// FP32 products in Kahan-compensated FP32 sums); kernel 2 forms S11 and S01 of
+// the requested model from those moments (demeaning and detrending are Schur
+// complements on the {1, u} columns) and solves the eigenproblem -- as one
+// kernel, or for the larger dims as two (algebra up to M = W^T W, then Jacobi).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -265,7 +267,7 @@ void walk_moments_fp64(const uint32_t* __restrict__ seeds, const double* __restr
 
 // ---------------------------------------------------------------------------
 // Kernel 2: per-walk model algebra + generalized eigen solve (one thread per
-// walk).  F_t = X_t - P Z_t with P = (sum X Z^T)(sum Z Z^T)^-1 applied to the
+// walk; split in two kernels where the fused one would fit too few waves).  F_t = X_t - P Z_t with P = (sum X Z^T)(sum Z Z^T)^-1 applied to the
 // projected rows of X:
 //   model 0: X = b,            no Z
 //   model 1: X = [b; 1],       no Z
