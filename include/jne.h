@@ -89,7 +89,9 @@ int jne_eigenvalues_by_seed(jne_ctx *ctx, int model, int dim, uint32_t steps, ui
 /* Same with device pointers (d_seeds, d_out in device memory of ctx's device);
  * the work is enqueued on `hip_stream` (a hipStream_t; NULL = the context's
  * stream) and the call returns without synchronising.  Non-positive-definite
- * walks yield NaN rows; no host-side check is made. */
+ * walks yield NaN rows; no host-side check is made.  The context's device
+ * scratch is shared by its calls: a call on another stream than the previous
+ * call first waits (on the device) for the previous call's use of it. */
 int jne_eigenvalues_by_seed_device(jne_ctx *ctx, int model, int dim, uint32_t steps, uint32_t rng_stream,
                                    const uint32_t *d_seeds, size_t n, int precision, double *d_out,
                                    void *hip_stream);

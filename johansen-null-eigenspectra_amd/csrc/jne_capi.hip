@@ -72,6 +72,10 @@ struct jne_ctx {
   // device scratch: M = W^T W of one chunk for the two-kernel eigen solve
   double* d_m = nullptr;
   size_t m_bytes = 0;
+  // the scratch is shared by every call: a call on another stream than the
+  // previous one first waits for the previous call's last use of it
+  hipEvent_t scratch_free = nullptr;
+  hipStream_t scratch_stream = nullptr;
   // device staging for the host-pointer entry points
   void* d_in = nullptr;
   size_t in_bytes = 0;
@@ -202,6 +206,9 @@ int run_walks(jne_ctx* ctx, int model, int dim, uint32_t steps, uint32_t rng_str
   if (rc) return rc;
   rc = ensure(ctx, (void**)&ctx->d_m, &ctx->m_bytes, chunk * (size_t)jne::eigen_m_count(dim) * sizeof(double));
   if (rc) return rc;
+  if (!ctx->scratch_free) JNE_HIP_CHECK(ctx, hipEventCreateWithFlags(&ctx->scratch_free, hipEventDisableTiming));
+  if (ctx->scratch_stream && ctx->scratch_stream != st)
+    JNE_HIP_CHECK(ctx, hipStreamWaitEvent(st, ctx->scratch_free, 0));
   for (size_t off = 0; off < n; off += chunk) {
     const uint32_t cn = (uint32_t)std::min(chunk, n - off);
     const uint32_t* s = d_seeds ? d_seeds + off : nullptr;
@@ -218,6 +225,8 @@ int run_walks(jne_ctx* ctx, int model, int dim, uint32_t steps, uint32_t rng_str
                                                        ctx->d_m, st);
                   }));
   }
+  JNE_HIP_CHECK(ctx, hipEventRecord(ctx->scratch_free, st));
+  ctx->scratch_stream = st;
   return JNE_OK;
 }
 
@@ -269,6 +278,7 @@ void jne_destroy(jne_ctx* ctx) {
   for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
   if (ctx->d_acc) (void)hipFree(ctx->d_acc);
   if (ctx->d_m) (void)hipFree(ctx->d_m);
+  if (ctx->scratch_free) (void)hipEventDestroy(ctx->scratch_free);
   if (ctx->d_in) (void)hipFree(ctx->d_in);
   if (ctx->d_out) (void)hipFree(ctx->d_out);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

@@ -82,6 +82,21 @@ class DeviceArray:
             self.hip().hipFree(self._p)
             self._p.value = None
 
+    @classmethod
+    def stream(cls) -> int:
+        """A new HIP stream (default priority) as an integer handle."""
+        import ctypes
+
+        st = ctypes.c_void_p()
+        assert cls.hip().hipStreamCreate(ctypes.byref(st)) == 0
+        return st.value
+
+    @classmethod
+    def destroy_stream(cls, st: int):
+        import ctypes
+
+        cls.hip().hipStreamDestroy(ctypes.c_void_p(st))
+
 
 @pytest.fixture(scope="session")
 def O():

@@ -96,6 +96,35 @@ def test_by_seed_matches_oracle_large(engine, O, model, dim, steps):
     assert_eigs_close(got, ref, RTOL, f"model {model} dim {dim} steps {steps}")
 
 
+def test_device_calls_on_two_streams(engine, jne):
+    """Device calls on one context alternating between two caller streams
+    (several chunks each, enqueued without synchronising): the context's
+    shared scratch is ordered between them, so both results are exact."""
+    from conftest import DeviceArray
+
+    model, dim, steps, n = 4, 12, 300, (1 << 18) + 5000
+    seeds = [np.arange(1 + k * n, 1 + (k + 1) * n, dtype=np.uint32) for k in range(2)]
+    streams = [DeviceArray.stream(), DeviceArray.stream()]
+    d_seeds = [DeviceArray(n * 4), DeviceArray(n * 4)]
+    d_out = [DeviceArray(n * dim * 8), DeviceArray(n * dim * 8)]
+    try:
+        for k in range(2):
+            d_seeds[k].from_host(seeds[k])
+        for _ in range(3):
+            for k in range(2):
+                engine.eigenvalues_by_seed_device(model, dim, steps, d_seeds[k].ptr, n, d_out[k].ptr,
+                                                  precision=jne.PRECISIONS["mfma_fp16x2"], hip_stream=streams[k])
+        got = [d_out[k].to_host((n, dim), np.float64) for k in range(2)]
+    finally:
+        for a in d_seeds + d_out:
+            a.free()
+        for st in streams:
+            DeviceArray.destroy_stream(st)
+    for k in range(2):
+        ref = engine.eigenvalues_by_seed(model, dim, steps, seeds[k], precision=jne.PRECISIONS["mfma_fp16x2"])
+        assert np.array_equal(got[k], ref), k
+
+
 @pytest.mark.parametrize("model,dim,steps,n", [(0, 2, 1000, 20000), (2, 5, 5000, 2000), (4, 12, 10000, 400)])
 def test_workload_shapes_many_walks(engine, O, model, dim, steps, n):
     """The plain-CPU, plain-GPU and headline workload shapes on many walks
