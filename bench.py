@@ -233,6 +233,79 @@ def stream_rate(pkg, eng, dim, T, torch, stream, walks=16384, reps=5):
     return walks * T * dim / (avg_ms * 1e-3), avg_ms
 
 
+def kernel_name(precision: str, dim: int) -> str:
+    if precision == "fp64":
+        return f"walk_moments_fp64<{dim}>"
+    if precision == "fp32_kahan":
+        return f"walk_moments_fp64<{dim}, kahan>"
+    return f"walk_moments_mfma<{dim}, {precision}>"
+
+
+def moment_roofline(precision, model, dim, T, mom, wps):
+    """The roofline object of the moment kernel: algorithmic FLOP per launch
+    over its HIP-event launch time against the unit's peak; HBM bytes per
+    launch from the committed counter runs (per-walk, copy-calibrated) times
+    this run's walks per launch; the per-unit rates from the committed
+    instruction counts."""
+    mom_ms, mom_launches, mom_walks = mom
+    fpw = algorithmic_flops_per_walk(model, dim, T)
+    avg_ms = mom_ms / max(mom_launches, 1)
+    walks_per_launch = mom_walks / max(mom_launches, 1)
+    achieved = fpw * walks_per_launch / (avg_ms * 1e-3) / 1e12
+    peak = PEAK_TFLOPS[precision]
+    key = f"walk_moments_{precision}_m{model}_d{dim}_T{T}"
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            rec = json.load(f).get(key)
+        if rec:
+            traffic = (rec["fetch_bytes_per_walk"] + rec["write_bytes_per_walk"]) * walks_per_launch
+    roofline = {
+        "bound": "mfma",
+        "achieved": achieved,
+        "peak": peak,
+        "unit": "TFLOP/s",
+        "frac": achieved / peak,
+        "traffic": traffic,
+        "traffic_unit": "bytes per launch (HBM, rocprofv3 FETCH_SIZE + WRITE_SIZE)",
+        "algorithmic_bytes_per_launch": (4 + 8 * (2 * dim * dim + 6 * dim)) * walks_per_launch,
+        "kernel": kernel_name(precision, dim),
+        "algorithmic_flop_per_walk": fpw,
+        "avg_launch_ms": avg_ms,
+        "walks_per_launch": walks_per_launch,
+        "normals_per_s": wps * T * dim,
+        "unit_rates": unit_rates_for(key, walks_per_launch, avg_ms),
+    }
+    if precision in MFMA_PER_CHUNK:
+        issued = MFMA_PER_CHUNK[precision] * 2 * 32 * 32 * 16 * ((T + 15) // 16)
+        roofline["issued_mfma_tflops"] = issued * walks_per_launch / (avg_ms * 1e-3) / 1e12
+    return roofline
+
+
+def cpu_baseline_block(model, dim, T, threads, head, fp64_head, k):
+    """cpu_baseline object: the C port on `threads` host threads over a bounded
+    sample, and its agreement with the GPU on the first walks."""
+    import numpy as np
+
+    rate, cn, cdt, ceig = cpu_baseline(model, dim, T, threads)
+    block = {
+        "value": rate,
+        "unit": "walks/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{cn:,} walks (seeds 1..{cn:,}) of the same workload in {cdt:.1f} s; C restatement of "
+                  f"the reference CPU walk (oracle/jne_oracle.c), one walk per thread",
+    }
+    m = min(64, cn, k)
+    if fp64_head is not None:
+        d = np.abs(fp64_head[:m] - ceig[:m])
+        block["fp64_gpu_vs_cpu_max_rel_diff_first_64"] = float(
+            (d / (np.abs(ceig[:m]) + 1e-13 * ceig[:m].max(axis=1, keepdims=True))).max())
+    block["headline_vs_cpu_max_rel_diff_first_64"] = float(relative_floor(head[:m], ceig[:m]).max())
+    return block
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -297,47 +370,8 @@ def main() -> int:
     head = out.view(n, nf)[:k].cpu().numpy()
     assert np.all(np.isfinite(head)), "non-finite eigenvalues"
 
-    fpw = algorithmic_flops_per_walk(model, dim, T)
-    mom_ms, mom_launches, mom_walks = mom
-    avg_ms = mom_ms / max(mom_launches, 1)
-    walks_per_launch = mom_walks / max(mom_launches, 1)
-    achieved = fpw * walks_per_launch / (avg_ms * 1e-3) / 1e12
-    peak = PEAK_TFLOPS[args.precision]
-    kernel = (f"walk_moments_fp64<{dim}>" if args.precision == "fp64"
-              else f"walk_moments_fp64<{dim}, kahan>" if args.precision == "fp32_kahan"
-              else f"walk_moments_mfma<{dim}, {args.precision}>")
-
-    # HBM bytes per launch from the committed counter runs (per-walk figures,
-    # copy-calibrated) times this run's walks per launch
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic.json")
-    key = f"walk_moments_{args.precision}_m{model}_d{dim}_T{T}"
-    if os.path.exists(tfile):
-        with open(tfile) as f:
-            rec = json.load(f).get(key)
-        if rec:
-            traffic = (rec["fetch_bytes_per_walk"] + rec["write_bytes_per_walk"]) * walks_per_launch
-
-    unit_rates = unit_rates_for(f"walk_moments_{args.precision}_m{model}_d{dim}_T{T}", walks_per_launch, avg_ms)
-    roofline = {
-        "bound": "mfma",
-        "achieved": achieved,
-        "peak": peak,
-        "unit": "TFLOP/s",
-        "frac": achieved / peak,
-        "traffic": traffic,
-        "traffic_unit": "bytes per launch (HBM, rocprofv3 FETCH_SIZE + WRITE_SIZE)",
-        "algorithmic_bytes_per_launch": (4 + 8 * (2 * dim * dim + 6 * dim)) * walks_per_launch,
-        "kernel": kernel,
-        "algorithmic_flop_per_walk": fpw,
-        "avg_launch_ms": avg_ms,
-        "walks_per_launch": walks_per_launch,
-        "normals_per_s": wps * T * dim,
-        "unit_rates": unit_rates,
-    }
-    if args.precision in MFMA_PER_CHUNK:
-        issued = MFMA_PER_CHUNK[args.precision] * 2 * 32 * 32 * 16 * ((T + 15) // 16)
-        roofline["issued_mfma_tflops"] = issued * walks_per_launch / (avg_ms * 1e-3) / 1e12
+    mom_ms, mom_launches, _ = mom
+    roofline = moment_roofline(args.precision, model, dim, T, mom, wps)
 
     line = {
         "metric": METRIC,
@@ -403,21 +437,7 @@ def main() -> int:
 
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        rate, cn, cdt, ceig = cpu_baseline(model, dim, T, threads)
-        line["cpu_baseline"] = {
-            "value": rate,
-            "unit": "walks/s",
-            "cores": threads,
-            "kind": "port",
-            "sample": f"{cn:,} walks (seeds 1..{cn:,}) of the same workload in {cdt:.1f} s; C restatement of "
-                      f"the reference CPU walk (oracle/jne_oracle.c), one walk per thread",
-        }
-        m = min(64, cn, k)
-        if fp64_head is not None:
-            d = np.abs(fp64_head[:m] - ceig[:m])
-            line["cpu_baseline"]["fp64_gpu_vs_cpu_max_rel_diff_first_64"] = float(
-                (d / (np.abs(ceig[:m]) + 1e-13 * ceig[:m].max(axis=1, keepdims=True))).max())
-        line["cpu_baseline"]["headline_vs_cpu_max_rel_diff_first_64"] = float(relative_floor(head[:m], ceig[:m]).max())
+        line["cpu_baseline"] = cpu_baseline_block(model, dim, T, threads, head, fp64_head, k)
     print(json.dumps(line), flush=True)
     eng.close()
     return 0
