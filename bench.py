@@ -13,11 +13,15 @@ state, model algebra and eigen solve).  The plain FP64 path is timed on the
 same batch in the same run, and the two outputs are compared walk by walk.
 
 Prints one JSON line with walks/s, the roofline figure of the dominant kernel
-(timed with HIP events on the stream it runs on), the rate of the random stream
-alone at the same shape (the normals every walk path must draw: the ceiling the
-moment kernel is measured against in roofline.frac_of_stream) and the CPU
-baseline (the C restatement of the reference walk in oracle/, on a bounded
-sample of the same workload, on this host's cores).
+(timed with HIP events on the stream it runs on), two rates of the normal
+generator on the same GPU and the CPU baseline (the C restatement of the
+reference walk in oracle/, on a bounded sample of the same workload, on this
+host's cores).  The generator rates: roofline.rng_normals_per_s is the
+generator alone (bin/jne_rng_rate: every normal drawn, nothing stored), the
+ceiling of a walk path that draws the same normals, and frac_of_rng is the
+moment kernel's normals/s against it; roofline.stream_normals_per_s is the
+normals stream of jne_walk_normals_device, which writes 4 B per normal to HBM
+and is bound by those writes (frac_of_stream against it).
 """
 from __future__ import annotations
 
@@ -233,6 +237,18 @@ def stream_rate(pkg, eng, dim, T, torch, stream, walks=16384, reps=5):
     return walks * T * dim / (avg_ms * 1e-3), avg_ms
 
 
+def rng_rate(timeout_s=120):
+    """Normals/s of the generator alone, from bin/jne_rng_rate (a child process
+    on the same GPU): the JSON line it prints, or None if it is not built."""
+    import subprocess
+
+    exe = os.path.join(ROOT, "johansen-null-eigenspectra_amd", "bin", "jne_rng_rate")
+    if not os.path.exists(exe):
+        return None
+    res = subprocess.run([exe], capture_output=True, text=True, timeout=timeout_s, check=True)
+    return json.loads(res.stdout.strip().splitlines()[-1])
+
+
 def kernel_name(precision: str, dim: int) -> str:
     if precision == "fp64":
         return f"walk_moments_fp64<{dim}>"
@@ -319,6 +335,7 @@ def main() -> int:
     ap.add_argument("--compare", default="fp64", help="path timed beside it ('' to skip)")
     ap.add_argument("--check-walks", type=int, default=10_000, help="walks compared between the two paths")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-rng-rate", action="store_true", help="skip the generator-alone rate (bin/jne_rng_rate)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cores)")
     ap.add_argument("--serve-rates", default="10,30,100,300,1000",
                     help="Poisson query rates (req/s) for the serving sweep ('' to skip)")
@@ -403,6 +420,10 @@ def main() -> int:
     roofline["stream_normals_per_s"] = srate
     roofline["stream_write_GBps"] = srate * 4 / 1e9
     roofline["frac_of_stream"] = roofline["normals_per_s"] / srate
+    rng = None if args.no_rng_rate else rng_rate()
+    roofline["rng_normals_per_s"] = rng["normals_per_s"] if rng else None
+    roofline["frac_of_rng"] = roofline["normals_per_s"] / rng["normals_per_s"] if rng else None
+    roofline["rng_source"] = ("bin/jne_rng_rate: " + json.dumps(rng)) if rng else "not measured in this run"
 
     if args.compare:
         cwps, csps, cmom, ceig_ms = time_path(pkg, eng, args.compare, model, dim, T, seeds, out, n, args.steps,
@@ -417,6 +438,7 @@ def main() -> int:
             "moments_avg_launch_ms": c_avg,
             "unit_rates": unit_rates_for(f"walk_moments_{args.compare}_m{model}_d{dim}_T{T}",
                                          cmom[2] / max(cmom[1], 1), c_avg),
+            "frac_of_rng": cwps * T * dim / rng["normals_per_s"] if rng else None,
             "speedup": wps / cwps,
             "max_rel_diff": float(rel.max()),
             "mean_rel_diff": float(rel.mean()),

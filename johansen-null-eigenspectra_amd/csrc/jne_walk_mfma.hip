@@ -42,10 +42,11 @@
 #include "jne_internal.h"
 #include "jne_rng.h"
 
-// Tuning: unroll the Philox calls of a segment (independent chains for the
-// scheduler, at the cost of registers); target waves per SIMD.
+// Tuning: how many of a lane's Philox calls of a segment to unroll
+// (independent chains for the scheduler, at the cost of registers: beyond 1
+// the FP64 Gram no longer fits beside them); target waves per SIMD.
 #ifndef JNE_MFMA_UNROLL_CALLS
-#define JNE_MFMA_UNROLL_CALLS 1
+#define JNE_MFMA_UNROLL_CALLS 2
 #endif
 #ifndef JNE_MFMA_WAVES
 #define JNE_MFMA_WAVES 4
@@ -101,6 +102,15 @@ struct MfmaShape {
 // format), lo = x - hi is exact in FP32 and rounded once to the narrow format.
 __device__ __forceinline__ float trunc_bits(float x, uint32_t mask) {
   return __uint_as_float(__float_as_uint(x) & mask);
+}
+
+// A wave-uniform double in scalar registers (VALU instructions read them as
+// operands), keeping two vector registers free.
+__device__ __forceinline__ double uniform_f64(double x) {
+  const uint64_t b = (uint64_t)__double_as_longlong(x);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
 // MFMA of one 16-step chunk.  x[8] are this lane's 8 row values (steps 8h + k,
@@ -198,10 +208,10 @@ struct ChunkMfma<JNE_PREC_MFMA_FP32> {
 // the three-way bf16 split, whose operands leave no registers for it
 template <int D, int PREC, bool FROM_NORMALS>
 struct CallUnroll {
-  static constexpr int value = (JNE_MFMA_UNROLL_CALLS && !FROM_NORMALS && PREC != JNE_PREC_MFMA_BF16X3 &&
-                                MfmaShape<D>::CALLS % 64 == 0)
-                                   ? MfmaShape<D>::IT
-                                   : 1;
+  static constexpr int value =
+      (JNE_MFMA_UNROLL_CALLS > 1 && !FROM_NORMALS && PREC != JNE_PREC_MFMA_BF16X3 && MfmaShape<D>::CALLS % 64 == 0)
+          ? (MfmaShape<D>::IT < JNE_MFMA_UNROLL_CALLS ? MfmaShape<D>::IT : JNE_MFMA_UNROLL_CALLS)
+          : 1;
 };
 
 template <bool B>
@@ -243,6 +253,7 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
   __shared__ double b0s[D];      // B at the segment start (FP64)
   __shared__ double seg_b0c[D];  // B0 + s m: the state the centred c rows are relative to
   __shared__ double mb[3 * D];   // sum over segments of B0' sum_seg u^p (lanes l < 3D)
+  __shared__ double gd[2 * D][3];  // Gram rows 24..26 of the c columns, then of the z columns
 
   const int l = threadIdx.x;
   const int row = l & 31;  // also the accumulator column
@@ -272,11 +283,15 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
 
   // FP64 Gram in registers: g[v] sums accumulator v of every segment, Gram
   // element [8(v/4) + 4h + v%4][l%32].  The output needs, on a c column, rows
-  // 0..26 (v = 0..11 on both halves, v = 12..14 on the lower half: rows
-  // 24..26) and, on a z column, rows 24..26 (v = 12..14, lower half).
-  double g[15];
+  // 0..23 (v = 0..11 on both halves) and rows 24..26 (v = 12..14 on the
+  // lower half), and on a z column rows 24..26; rows 24..26 are summed in LDS.
+  double g[12];
 #pragma unroll
-  for (int v = 0; v < 15; ++v) g[v] = 0.0;
+  for (int v = 0; v < 12; ++v) g[v] = 0.0;
+  // rows 24..26 (v = 12..14 on the lower half) of the c and z columns: in LDS
+  const bool det_lane = h == 0 && (is_c || is_z);
+  double* const gdl = gd[is_c ? row - kRowC : D + (row < D ? row : 0)];
+  for (int e = l; e < 6 * D; e += 64) gd[e / 3][e % 3] = 0.0;
 
   // (i, j) grid entries of this lane: e = l + 64 g
   int gi[NG], gj[NG];
@@ -297,8 +312,10 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
 
   if (l < D) b0s[l] = 0.0;
   if (l < 3 * D) mb[l] = 0.0;
-  const double dt = 1.0 / (double)T;
-  const double s = sqrt(dt);
+  // walk constants: uniform, held in scalar registers
+  const double dt = uniform_f64(1.0 / (double)T);
+  const double s = uniform_f64(sqrt(dt));
+  const double dt2 = uniform_f64(dt * dt);
   const uint32_t rounds = (T + kSegSteps - 1) / kSegSteps;
 
   // one segment; the partial last segment (T not a multiple of 64) runs a copy
@@ -402,9 +419,11 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
       const double P1 = (double)f[13], P2 = (double)f[14];
 #pragma unroll
       for (int v = 0; v < 12; ++v) g[v] += (double)f[v];
-      g[12] += P0;
-      g[13] += (a * P0 + P1) * dt;
-      g[14] += ((a * a) * P0 + (2.0 * a + 1.0) * P1 + 2.0 * P2) * (dt * dt);
+      if (det_lane) {
+        gdl[0] += P0;
+        gdl[1] += (a * P0 + P1) * dt;
+        gdl[2] += ((a * a) * P0 + (2.0 * a + 1.0) * P1 + 2.0 * P2) * dt2;
+      }
     }
     // FP64 per-segment terms on the (i, j) grid
 #pragma unroll
@@ -426,7 +445,7 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
       const double A = (double)ts + 1.0;
       const double k1 = (double)(n * (n - 1) / 2), k2 = (double)(n * (n - 1) * (2 * n - 1) / 6);
       const double s1 = (nseg * A + k1) * dt;
-      const double s2 = ((nseg * A) * A + (2.0 * A) * k1 + k2) * (dt * dt);
+      const double s2 = ((nseg * A) * A + (2.0 * A) * k1 + k2) * dt2;
       const double up = mp == 0 ? nseg : mp == 1 ? s1 : s2;
       mb[l] += seg_b0c[mi] * up;
     }
@@ -438,22 +457,31 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
   for (uint32_t r = 0; r < full; ++r) segment(r, BoolTag<false>{});
   if (full < rounds) segment(full, BoolTag<true>{});
 
+  // Lane indices of the output, recomputed from an opaque copy of the lane id
+  // (an empty asm statement: no instruction), so that the compiler does not
+  // keep the ones computed before the segment loop alive across it.
+  int lx = l;
+  asm volatile("" : "+v"(lx));
+  const int xrow_ = lx & 31, xh = lx >> 5;
+  const bool xc = xrow_ >= kRowC && xrow_ < kRowC + D, xz = xrow_ < D;
+
   // the Gram entries the output reads, into LDS over the row buffers (their
   // last reads came before the last segment's barriers): c lanes rows 0..23 of
   // their column, the lower half also rows 24..26; z lanes rows 24..26
-  if (is_c) {
-    double2* const gcol = reinterpret_cast<double2*>(gram + (row - kRowC) * kGramPitch + 4 * h);
+  if (xc) {
+    double2* const gcol = reinterpret_cast<double2*>(gram + (xrow_ - kRowC) * kGramPitch + 4 * xh);
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       gcol[4 * q] = make_double2(g[4 * q], g[4 * q + 1]);
       gcol[4 * q + 1] = make_double2(g[4 * q + 2], g[4 * q + 3]);
     }
   }
-  if (h == 0 && (is_c || is_z)) {
-    double* const gdet = is_c ? gram + (row - kRowC) * kGramPitch + kRowDet : gram + kGramZ + 3 * row;
-    gdet[0] = g[12];
-    gdet[1] = g[13];
-    gdet[2] = g[14];
+  if (xh == 0 && (xc || xz)) {
+    double* const gdet = xc ? gram + (xrow_ - kRowC) * kGramPitch + kRowDet : gram + kGramZ + 3 * xrow_;
+    const double* const src = gd[xc ? xrow_ - kRowC : D + xrow_];
+    gdet[0] = src[0];
+    gdet[1] = src[1];
+    gdet[2] = src[2];
   }
   __syncthreads();
 
@@ -462,17 +490,18 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
   // sum b, sum u b, sum u^2 b, sum e, sum u e, sum u^2 e
 #pragma unroll
   for (int q = 0; q < NG; ++q) {
-    const int e = l + 64 * q;
+    const int e = lx + 64 * q;
     if (e < D * D) {
-      const int i = gi[q], j = gj[q];
+      const int i = e / D, j = e % D;
       const int lo = i < j ? i : j, hi = i < j ? j : i;  // one triangle: exactly symmetric
       acc[(size_t)e * ld + walk] = bbA[q] + s2 * gram[hi * kGramPitch + kRowC + lo];
       acc[(size_t)(D * D + e) * ld + walk] = ebA[q] + s2 * gram[j * kGramPitch + i];
     }
   }
-  if (l < 3 * D) {
-    acc[(size_t)(2 * D * D + mp * D + mi) * ld + walk] = mb[l] + s * gram[mi * kGramPitch + kRowDet + mp];
-    acc[(size_t)(2 * D * D + (3 + mp) * D + mi) * ld + walk] = s * gram[kGramZ + 3 * mi + mp];
+  if (lx < 3 * D) {
+    const int xp = lx / D, xi = lx % D;
+    acc[(size_t)(2 * D * D + xp * D + xi) * ld + walk] = mb[lx] + s * gram[xi * kGramPitch + kRowDet + xp];
+    acc[(size_t)(2 * D * D + (3 + xp) * D + xi) * ld + walk] = s * gram[kGramZ + 3 * xi + xp];
   }
 }
 
