@@ -1,11 +1,14 @@
 """bench.py on the CPU: the one-GPU contract and the helpers that shape its
-JSON line (no GPU work)."""
+JSON line (no GPU work); and bin/jne_rng_rate, the generator-alone rate that
+bench.py reports beside the moment kernel (argument checks on the CPU, its
+arithmetic against the oracle's stream on the GPU)."""
 import json
 import os
 import subprocess
 import sys
 
 import numpy as np
+import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, ROOT)
@@ -41,3 +44,42 @@ def test_algorithmic_flops():
     # 2 T nF (nF + d): the reference's S11 and S01 contractions
     assert bench.algorithmic_flops_per_walk(4, 12, 10000) == 2 * 10000 * 12 * 24
     assert bench.algorithmic_flops_per_walk(1, 3, 100) == 2 * 100 * 4 * 7
+
+
+RNG_RATE = os.path.join(ROOT, "johansen-null-eigenspectra_amd", "bin", "jne_rng_rate")
+needs_tool = pytest.mark.skipif(not os.path.exists(RNG_RATE), reason="bin/jne_rng_rate is not built")
+
+
+@needs_tool
+@pytest.mark.parametrize("args", [["--calls", "4"], ["--calls", "0"], ["--blocks", "0"], ["--reps", "101"],
+                                  ["--bogus", "1"]])
+def test_rng_rate_tool_rejects_bad_arguments(args):
+    # checked before any HIP call: exit 2 with a message, no GPU needed
+    res = subprocess.run([RNG_RATE] + args, capture_output=True, text=True, timeout=60)
+    assert res.returncode == 2
+    assert "jne_rng_rate" in res.stderr
+
+
+@needs_tool
+@pytest.mark.gpu
+def test_rng_rate_tool_draws_the_walk_normals(O):
+    """Thread t of the tool draws the normals of walk t + 1 (stream 0) in
+    order and sums them in FP32, (v0 + v1) + (v2 + v3) per Philox call; the
+    printed total (a double sum over threads) is replayed from the oracle's
+    stream with the same FP32 operations."""
+    blocks, calls = 2, 30
+    res = subprocess.run([RNG_RATE, "--blocks", str(blocks), "--calls", str(calls), "--reps", "2"],
+                         capture_output=True, text=True, timeout=120, check=True)
+    out = json.loads(res.stdout.strip().splitlines()[-1])
+    threads = blocks * 256
+    assert out["threads"] == threads and out["calls_per_thread"] == calls
+    assert out["normals_per_launch"] == 4 * threads * calls
+    assert out["normals_per_s"] > 0
+    total = 0.0
+    for t in range(threads):
+        v = O.walk_normals(t + 1, 0, 1, 4 * calls).reshape(calls, 4).astype(np.float32)
+        acc = np.float32(0.0)
+        for q in range(calls):
+            acc = np.float32(acc + np.float32(np.float32(v[q, 0] + v[q, 1]) + np.float32(v[q, 2] + v[q, 3])))
+        total += float(acc)
+    assert abs(out["sum_of_outputs"] - total) <= 5e-7, (out["sum_of_outputs"], total)
