@@ -89,11 +89,12 @@ struct MfmaShape {
   static constexpr int IT = (CALLS + 63) / 64;          // call iterations per lane
   static constexpr int ZP = kSegSteps + 4;              // LDS pitch of one row (floats)
   static constexpr int NG = (D * D + 63) / 64;          // (i, j) grid entries per lane
-  static constexpr int SP = 64 / D < 8 ? 64 / D : 8;    // scan lanes per component
-  static constexpr int BS = (kSegSteps + SP - 1) / SP;  // steps per scan lane
-  // LDS rows z (D), 1, k, k(k-1)/2 and c (D); the compact FP64 Gram takes the
-  // same bytes after the last segment
-  static constexpr int ROW_BYTES = (2 * D + 3) * ZP * 4;
+  static constexpr int SP = D <= 8 ? 8 : 4;             // scan lanes per component
+  static constexpr int BS = kSegSteps / SP;             // steps per scan lane: whole 16-byte units
+  // LDS rows, each at its row index of X: z (0..D-1), c (12..12+D-1), 1, k,
+  // k(k-1)/2 (24..26); the compact FP64 Gram takes the same bytes after the
+  // last segment
+  static constexpr int ROW_BYTES = (kRowDet + 3) * ZP * 4;
   static constexpr int RAW_BYTES = ROW_BYTES > kGramSize * 8 ? ROW_BYTES : kGramSize * 8;
   static_assert(D <= kRowC, "dim must fit the z block");
 };
@@ -240,12 +241,15 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
   constexpr int ZP = Sh::ZP, NG = Sh::NG, SP = Sh::SP, BS = Sh::BS;
   using Op = ChunkMfma<PREC>;
 
-  // rows [0, D): the segment's normals z (component-major); [D, D+3): the
-  // constant rows 1, k, k(k-1)/2, written once; [D+3, 2D+3): the c rows of
-  // the segment.  After the last segment the same bytes hold the FP64 Gram.
+  // Row r of X lives in LDS row r: the segment's normals z (rows 0..D-1), its
+  // c rows (12..12+D-1) and the constant rows 1, k, k(k-1)/2 (24..26), written
+  // once.  With a pitch of 17 16-byte units, the 8 lanes of a 128-bit read
+  // group (8 consecutive rows) fall in distinct banks.  After the last
+  // segment the same bytes hold the FP64 Gram.
   __shared__ __align__(16) unsigned char rows_raw[Sh::RAW_BYTES];
-  float(*const zbuf)[ZP] = reinterpret_cast<float(*)[ZP]>(rows_raw);
-  float(*const cbuf)[ZP] = zbuf + D + 3;
+  float(*const xs)[ZP] = reinterpret_cast<float(*)[ZP]>(rows_raw);
+  float(*const zbuf)[ZP] = xs;
+  float(*const cbuf)[ZP] = xs + kRowC;
   double* const gram = reinterpret_cast<double*>(rows_raw);
   __shared__ double eps_part[SP][D];
   __shared__ double seg_eps[D];
@@ -266,19 +270,20 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
 
   const bool is_z = row < D;
   const bool is_c = row >= kRowC && row < kRowC + D;
-  // LDS row feeding this lane's row of X.  The rows of unused components
-  // (dim < 12) and rows 27..31 read z row 0: their Gram entries are never read.
-  const float* const xrow = is_z   ? zbuf[row]
-                            : is_c ? cbuf[row - kRowC]
-                            : (row >= kRowDet && row < kRowDet + 3) ? zbuf[D + row - kRowDet]
-                                                                    : zbuf[0];
+  // LDS row feeding this lane's row of X: its own; rows 27..31 read rows
+  // 19..23 (distinct banks from rows 24..26 in their read group).  Their Gram
+  // entries, and those of the rows of unused components (dim < 12, zero), are
+  // never read.
+  const float* const xrow = xs[row < kRowDet + 3 ? row : row - 8];
 
-  // constant rows, exact small integers (k(k-1)/2 <= 2016)
+  // every row zero (the rows of unused components stay so), then the constant
+  // rows, exact small integers (k(k-1)/2 <= 2016)
+  for (int e = l; e < (kRowDet + 3) * ZP; e += 64) xs[e / ZP][e % ZP] = 0.0f;
   for (int t = l; t < ZP; t += 64) {
     const float kf = (float)t;
-    zbuf[D][t] = 1.0f;
-    zbuf[D + 1][t] = kf;
-    zbuf[D + 2][t] = kf * (kf - 1.0f) * 0.5f;
+    xs[kRowDet][t] = 1.0f;
+    xs[kRowDet + 1][t] = kf;
+    xs[kRowDet + 2][t] = kf * (kf - 1.0f) * 0.5f;
   }
 
   // FP64 Gram in registers: g[v] sums accumulator v of every segment, Gram
@@ -350,15 +355,22 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
     // ---- scan, part 1: lane (si, sp) sums its block of component si in FP64:
     // the partial sums give the segment's eps, which drives the FP64 state,
     // and the offsets of the c rows in the blocks after it
+    // (128-bit LDS reads and writes: the 8 lanes of a read group cover
+    // distinct banks)
     float zb[BS];
     if (scan_lane) {
+      const float4* const src = reinterpret_cast<const float4*>(&zbuf[si][sp * BS]);
+#pragma unroll
+      for (int q = 0; q < BS / 4; ++q) {
+        const float4 v = src[q];
+        zb[4 * q] = v.x;
+        zb[4 * q + 1] = v.y;
+        zb[4 * q + 2] = v.z;
+        zb[4 * q + 3] = v.w;
+      }
       double e = 0.0;
 #pragma unroll
-      for (int j = 0; j < BS; ++j) {
-        const int t = sp * BS + j;
-        zb[j] = (SP * BS == kSegSteps || t < kSegSteps) ? zbuf[si][t] : 0.0f;
-        e += (double)zb[j];
-      }
+      for (int j = 0; j < BS; ++j) e += (double)zb[j];
       eps_part[sp][si] = e;
     }
     __syncthreads();
@@ -377,11 +389,17 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
       double c = -(double)m;
 #pragma unroll
       for (int p = 0; p < SP - 1; ++p) c += p < sp ? eps_part[p][si] : 0.0;
+      float4* const dst = reinterpret_cast<float4*>(&cbuf[si][sp * BS]);
 #pragma unroll
-      for (int j = 0; j < BS; ++j) {
-        const int t = sp * BS + j;
-        if (SP * BS == kSegSteps || t < kSegSteps) cbuf[si][t] = (!tail || ts + (uint32_t)t < T) ? (float)c : 0.0f;
-        c += (double)zb[j];
+      for (int q = 0; q < BS / 4; ++q) {
+        float cq[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const uint32_t t = ts + (uint32_t)(sp * BS + 4 * q + k);
+          cq[k] = (!tail || t < T) ? (float)c : 0.0f;
+          c += (double)zb[4 * q + k];
+        }
+        dst[q] = make_float4(cq[0], cq[1], cq[2], cq[3]);
       }
       if (sp == 0) {
         seg_eps[si] = e;
