@@ -48,7 +48,7 @@ extern "C" {
  * and put only within-segment corrections on the matrix cores.  Both paths
  * draw the same normals, whose generation bounds them both; measured on
  * MI355X the MFMA pass is faster for dims 11-12 from 300 steps and dims 9-10
- * from 400 steps, so JNE_PREC_AUTO runs MFMA_FP16X2 there and FP64
+ * from 500 steps, so JNE_PREC_AUTO runs MFMA_FP16X2 there and FP64
  * elsewhere (FP64 results meet the mixed-path tolerance trivially). */
 
 #define JNE_MAX_DIM 12

@@ -276,9 +276,15 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
   // never read.
   const float* const xrow = xs[row < kRowDet + 3 ? row : row - 8];
 
-  // every row zero (the rows of unused components stay so), then the constant
-  // rows, exact small integers (k(k-1)/2 <= 2016)
-  for (int e = l; e < (kRowDet + 3) * ZP; e += 64) xs[e / ZP][e % ZP] = 0.0f;
+  // the rows of unused components (dim < 12) zero, then the constant rows,
+  // exact small integers (k(k-1)/2 <= 2016)
+  if constexpr (D < kRowC) {
+    for (int r = D; r < kRowC; ++r)
+      for (int t = l; t < ZP; t += 64) {
+        xs[r][t] = 0.0f;
+        xs[kRowC + r][t] = 0.0f;
+      }
+  }
   for (int t = l; t < ZP; t += 64) {
     const float kf = (float)t;
     xs[kRowDet][t] = 1.0f;
