@@ -29,6 +29,7 @@
 #define JNE_FP64_UNROLL_CALLS 0
 #endif
 
+
 namespace jne {
 
 // ---------------------------------------------------------------------------
@@ -109,9 +110,13 @@ void walk_moments_fp64(const uint32_t* __restrict__ seeds, const double* __restr
   __shared__ double ws[S][RW][GP];
 
   const int lane = threadIdx.x;
-  const int g = lane / L;
-  const int r = lane - g * L;
-  const bool in_group = g < GPW;
+  // walk fastest across the lanes (lane = r * GPW + g): the 16 lanes of a
+  // 64-bit LDS access group read the same row of 16 different walks (odd walk
+  // pitch: distinct banks), and the output stores of a lane row go to
+  // consecutive walks.  One lane per walk: lane = walk.
+  const int r = L == 1 ? 0 : lane / GPW;
+  const int g = L == 1 ? lane : lane - r * GPW;
+  const bool in_group = L == 1 || r < L;
   const int gs = in_group ? g : 0;
   const size_t walk = (size_t)blockIdx.x * GPW + gs;
   const bool valid = in_group && walk < n_walks;
