@@ -120,6 +120,17 @@ int jne_walk_normals(jne_ctx *ctx, int dim, uint32_t steps, uint32_t rng_stream,
 int jne_walk_normals_device(jne_ctx *ctx, int dim, uint32_t steps, uint32_t rng_stream, const uint32_t *d_seeds,
                             size_t n, float *d_out, void *hip_stream);
 
+/* Percentiles p[0..np) (1 <= np <= 8, each in [0, 1]) of the device array
+ * d_values[0..n), n < 2^31, into the host array out[np]: rank p*(n-1),
+ * linear interpolation between the neighbouring order statistics, the same
+ * values as jne_percentiles, found on the device by bucket selection (a sort
+ * only when ties crowd the selected buckets).  Synchronous on hip_stream
+ * (NULL = the context's stream); n = 0 gives NaN as jne_percentiles.  The
+ * device counterpart of get_percentile_value (src/simulation_analyzers.rs:4-18),
+ * used by the query server; NaN values are JNE_ERR_NUMERIC. */
+int jne_percentiles_device(jne_ctx *ctx, const double *d_values, uint64_t n, const double *p, size_t np,
+                           double *out, void *hip_stream);
+
 /* Block until all work enqueued on the context's stream has finished. */
 int jne_synchronize(jne_ctx *ctx);
 

@@ -10,7 +10,8 @@ Mirrors the reference's interfaces around the walk path:
   of ``src/johansen_statistics.rs:59-85`` (returns |rho| sorted descending);
 * ``Engine`` -- the batched forms used in place of the per-seed rayon loop of
   ``src/data_storage/parallel_compute.rs:14-41`` (host or device pointers, by
-  seed or from caller-supplied normals, and the device stream itself);
+  seed or from caller-supplied normals, the device stream itself, and
+  percentiles of a device array by the rule of ``src/simulation_analyzers.rs:4-18``);
 * ``JohansenModel`` -- the model enum of ``src/johansen_models.rs:6-141``;
 * ``EigenvalueSimulation`` -- the library facade of
   ``src/data_storage/simulation.rs`` (runs streamed to the ``.dat`` V6 file
@@ -119,6 +120,9 @@ def load_library() -> ctypes.CDLL:
         L.jne_walk_normals_device.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32,
                                               ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p]
         L.jne_walk_normals_device.restype = ctypes.c_int
+        L.jne_percentiles_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p,
+                                             ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p]
+        L.jne_percentiles_device.restype = ctypes.c_int
         L.jne_synchronize.argtypes = [ctypes.c_void_p]
         L.jne_synchronize.restype = ctypes.c_int
         L.jne_profile_enable.argtypes = [ctypes.c_void_p, ctypes.c_int]
@@ -325,6 +329,15 @@ class Engine:
         self._check(self._lib.jne_walk_normals_device(
             self._ctx, int(dim), int(steps), int(rng_stream), ctypes.c_void_p(d_seeds), int(n),
             ctypes.c_void_p(d_out), ctypes.c_void_p(hip_stream or None)))
+
+    def percentiles_device(self, d_values: int, n: int, ps, hip_stream: int = 0) -> list:
+        """Percentiles ps (1..8 values in [0, 1]) of the device array d_values[0..n) (a device pointer,
+        int), selected on the device; the rule and values of percentiles()."""
+        p = np.ascontiguousarray(ps, dtype=np.float64)
+        out = np.empty(max(len(p), 1), dtype=np.float64)
+        self._check(self._lib.jne_percentiles_device(self._ctx, ctypes.c_void_p(d_values), int(n), p.ctypes.data,
+                                                     len(p), out.ctypes.data, ctypes.c_void_p(hip_stream or None)))
+        return [float(x) for x in out[:len(p)]]
 
     def synchronize(self):
         self._check(self._lib.jne_synchronize(self._ctx))

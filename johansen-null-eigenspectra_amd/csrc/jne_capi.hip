@@ -81,6 +81,8 @@ struct jne_ctx {
   size_t in_bytes = 0;
   void* d_out = nullptr;
   size_t out_bytes = 0;
+  // device scratch of jne_percentiles_device
+  jne::QuantileScratch qs;
   // kernel timing (jne_profile_*)
   struct TimedLaunch {
     int kernel;
@@ -397,6 +399,29 @@ int jne_walk_normals_device(jne_ctx* ctx, int dim, uint32_t steps, uint32_t rng_
   JNE_HIP_CHECK(ctx, timed_launch(ctx, JNE_KERNEL_NORMALS, n, st, [&] {
                   return jne::launch_walk_normals(dim, d_seeds, rng_stream, steps, (uint32_t)n, d_out, st);
                 }));
+  return JNE_OK;
+}
+
+int jne_percentiles_device(jne_ctx* ctx, const double* d_values, uint64_t n, const double* p, size_t np,
+                           double* out, void* hip_stream) {
+  if (!ctx) return JNE_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->err.clear();
+  if (!p || !out || np < 1 || np > (size_t)jne::kMaxPercentiles)
+    return set_err(ctx, JNE_ERR_INVALID_ARG, "p/out must not be NULL and np must be 1..%d", jne::kMaxPercentiles);
+  for (size_t i = 0; i < np; ++i)
+    if (!(p[i] >= 0.0 && p[i] <= 1.0)) return set_err(ctx, JNE_ERR_INVALID_ARG, "p[%zu] is not in [0, 1]", i);
+  if (n > 0x7FFFFFFFull) return set_err(ctx, JNE_ERR_INVALID_ARG, "n must be below 2^31");
+  if (n == 0) {  // as jne_percentiles: no order statistics
+    for (size_t i = 0; i < np; ++i) out[i] = NAN;
+    return JNE_OK;
+  }
+  if (!d_values) return set_err(ctx, JNE_ERR_INVALID_ARG, "d_values must not be NULL");
+  JNE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = hip_stream ? (hipStream_t)hip_stream : ctx->stream;
+  const int rc = jne::quantile_select(ctx->qs, &d_values, 1, n, p, (int)np, out, st, nullptr);
+  if (rc == JNE_ERR_NUMERIC) return set_err(ctx, rc, "a value is NaN");
+  if (rc) return set_err(ctx, rc, "percentile selection failed");
   return JNE_OK;
 }
 

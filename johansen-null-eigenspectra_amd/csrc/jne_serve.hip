@@ -13,14 +13,12 @@
 // A query that arrives while others run is admitted as soon as its HBM
 // footprint (2 doubles per walk for its per-walk statistics) fits the
 // budget, and its chunks interleave with the running queries' chunks on the
-// device.  When a query's last chunk is done a finisher thread sorts its two
-// statistics arrays on the device (radix sort, in the query's own memory plus
-// one alternate buffer), reads the three ranks of each on the device and
-// copies back six values.
+// device.  When a query's last chunk is done a finisher thread selects the
+// three quantiles of each of its two statistics arrays on the device (bucket
+// selection, jne_quantile.hip: no full sort) and copies back six values.
 #include <hip/hip_runtime.h>
 #include <math.h>
 
-#include <hipcub/hipcub.hpp>
 #include <stdio.h>
 #include <string.h>
 
@@ -37,6 +35,7 @@
 
 #include "../../include/jne.h"
 #include "jne_host.h"
+#include "jne_internal.h"
 
 namespace {
 
@@ -64,28 +63,6 @@ __global__ void walk_statistics(const double* __restrict__ eigs, int nf, uint32_
 }
 
 constexpr uint32_t kChunkWalks = 1u << 16;
-
-// The 0.90, 0.95 and 0.99 quantiles of an ascending array s[0..n), n >= 1:
-// rank p*(n-1), linear interpolation between its neighbours, in the
-// arithmetic of get_percentile_value (src/simulation_analyzers.rs:4-18).
-// out[3] = 1 if the array holds a NaN: radix order puts NaNs with the sign
-// bit clear after +inf and those with it set before -inf, so the ends tell.
-__global__ void sorted_quantiles(const double* __restrict__ s, uint64_t n, double* __restrict__ out) {
-  const int k = threadIdx.x;
-  if (k < 3) {
-    const double p = k == 0 ? 0.90 : k == 1 ? 0.95 : 0.99;
-    const double rank = p * (double)(n - 1);
-    const uint64_t lo = (uint64_t)floor(rank), hi = (uint64_t)ceil(rank);
-    double v = s[lo];
-    if (lo != hi) {
-      const double w = rank - (double)lo;
-      v = s[lo] * (1.0 - w) + s[hi] * w;
-    }
-    out[k] = v;
-  } else if (k == 3) {
-    out[3] = (isnan(s[0]) || isnan(s[n - 1])) ? 1.0 : 0.0;
-  }
-}
 
 struct Request {
   uint64_t id;
@@ -293,62 +270,12 @@ void scheduler_loop(jne_server* s) {
   }
 }
 
-// Device scratch of the finisher: the radix sort's alternate key buffer and
-// temporary storage, grown to the largest query seen.
-struct SortScratch {
-  double* alt = nullptr;
-  size_t alt_items = 0;
-  void* temp = nullptr;
-  size_t temp_bytes = 0;
-  double* d_q = nullptr;  // 2 x (3 quantiles + NaN flag)
-  ~SortScratch() {
-    if (alt) (void)hipFree(alt);
-    if (temp) (void)hipFree(temp);
-    if (d_q) (void)hipFree(d_q);
-  }
-};
-
-// Sorts trace[runs] and maxeig[runs] (the query's statistics, in place of the
-// arena memory) and leaves quantiles + NaN flag of each in q[0..8).
-hipError_t device_quantiles(SortScratch& sc, double* trace, double* maxeig, uint64_t runs, hipStream_t st,
-                            double q[8]) {
-  hipError_t e = hipSuccess;
-  if (!sc.d_q && (e = hipMalloc((void**)&sc.d_q, 8 * sizeof(double))) != hipSuccess) return e;
-  if (sc.alt_items < runs) {
-    if (sc.alt) (void)hipFree(sc.alt);
-    sc.alt = nullptr;
-    sc.alt_items = 0;
-    if ((e = hipMalloc((void**)&sc.alt, runs * sizeof(double))) != hipSuccess) return e;
-    sc.alt_items = runs;
-  }
-  size_t need = 0;
-  hipcub::DoubleBuffer<double> probe(trace, sc.alt);
-  if ((e = hipcub::DeviceRadixSort::SortKeys(nullptr, need, probe, (int)runs, 0, 64, st)) != hipSuccess) return e;
-  if (sc.temp_bytes < need) {
-    if (sc.temp) (void)hipFree(sc.temp);
-    sc.temp = nullptr;
-    sc.temp_bytes = 0;
-    if ((e = hipMalloc(&sc.temp, need)) != hipSuccess) return e;
-    sc.temp_bytes = need;
-  }
-  double* arrays[2] = {trace, maxeig};
-  for (int j = 0; j < 2; ++j) {
-    hipcub::DoubleBuffer<double> keys(arrays[j], sc.alt);
-    size_t tb = sc.temp_bytes;
-    if ((e = hipcub::DeviceRadixSort::SortKeys(sc.temp, tb, keys, (int)runs, 0, 64, st)) != hipSuccess) return e;
-    hipLaunchKernelGGL(sorted_quantiles, dim3(1), dim3(64), 0, st, keys.Current(), runs, sc.d_q + 4 * j);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-  }
-  if ((e = hipMemcpyAsync(q, sc.d_q, 8 * sizeof(double), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
-  return hipStreamSynchronize(st);
-}
-
 void finisher_loop(jne_server* s) {
   hipStream_t st = nullptr;
   (void)hipSetDevice(s->device);
   (void)hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
   {
-    SortScratch sc;
+    jne::QuantileScratch sc;
     std::unique_lock<std::mutex> lk(s->mu);
     while (true) {
       s->done_cv.wait(lk, [&] { return s->stop || !s->to_finish.empty(); });
@@ -357,14 +284,16 @@ void finisher_loop(jne_server* s) {
       s->to_finish.pop_front();
       double* const dev = reinterpret_cast<double*>(s->arena_base + r->arena_off);
       lk.unlock();
-      double q8[8];
-      const hipError_t e = device_quantiles(sc, dev, dev + r->runs, r->runs, st, q8);
-      int code = e == hipSuccess ? 0 : JNE_ERR_HIP;
-      std::string msg = e == hipSuccess ? "" : std::string("quantile selection: ") + hipGetErrorString(e);
-      if (code == 0 && (q8[3] != 0.0 || q8[7] != 0.0)) {
-        code = JNE_ERR_NUMERIC;
+      // the 0.90/0.95/0.99 quantiles of trace and max-eig, selected on the device
+      static const double ps[3] = {0.90, 0.95, 0.99};
+      double q6[6];
+      const double* const arrays[2] = {dev, dev + r->runs};
+      const int code = jne::quantile_select(sc, arrays, 2, r->runs, ps, 3, q6, st, nullptr);
+      std::string msg;
+      if (code == JNE_ERR_NUMERIC)
         msg = "a walk's S11 is not positive definite (NaN eigenvalues)";
-      }
+      else if (code)
+        msg = "quantile selection failed (error " + std::to_string(code) + ")";
       lk.lock();
       s->arena.free(r->arena_off);
       r->has_arena = false;
@@ -372,8 +301,7 @@ void finisher_loop(jne_server* s) {
       if (code) {
         fail_request(s, r, code, msg);
       } else {
-        const double q[6] = {q8[0], q8[1], q8[2], q8[4], q8[5], q8[6]};
-        memcpy(r->q, q, sizeof q);
+        memcpy(r->q, q6, sizeof q6);
         r->complete = true;
         r->finished = Clock::now();
         s->done_cv.notify_all();
