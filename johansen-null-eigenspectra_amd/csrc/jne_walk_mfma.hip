@@ -99,6 +99,11 @@ struct MfmaShape {
   static_assert(D <= kRowC, "dim must fit the z block");
 };
 
+// Per-component FP64 values of a segment (see the kernel).
+struct alignas(16) SegVec {
+  double P, b, Q, E;
+};
+
 // hi/lo splits by truncation: hi keeps the leading bits (exact in the narrow
 // format), lo = x - hi is exact in FP32 and rounded once to the narrow format.
 __device__ __forceinline__ float trunc_bits(float x, uint32_t mask) {
@@ -253,9 +258,11 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
   double* const gram = reinterpret_cast<double*>(rows_raw);
   __shared__ double eps_part[SP][D];
   __shared__ double seg_eps[D];
-  __shared__ double seg_sig[D];
-  __shared__ double b0s[D];      // B at the segment start (FP64)
-  __shared__ double seg_b0c[D];  // B0 + s m: the state the centred c rows are relative to
+  __shared__ double b0s[D];  // B at the segment start (FP64)
+  // per component j of a segment: b = B0 + s m (the state the centred c rows
+  // are relative to), P = n b + s sig, Q = s sig, E = s eps, so that the
+  // segment adds b_i P_j + Q_i b_j to sum b b^T and E_i b_j to sum e b^T
+  __shared__ SegVec seg[D];
   __shared__ double mb[3 * D];   // sum over segments of B0' sum_seg u^p (lanes l < 3D)
   __shared__ double gd[2 * D][3];  // Gram rows 24..26 of the c columns, then of the z columns
 
@@ -409,7 +416,8 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
       }
       if (sp == 0) {
         seg_eps[si] = e;
-        seg_b0c[si] = b0s[si] + s * (double)m;
+        seg[si].b = b0s[si] + s * (double)m;
+        seg[si].E = s * e;
       }
     }
     __syncthreads();
@@ -430,10 +438,15 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
 #pragma unroll
     for (int v = 0; v < 16; ++v) f[v] = accm[v] + accc[v];
     // f[12] on the lower half is row 24 (the ones row): sum_seg x_col
-    if (h == 0 && is_c) seg_sig[row - kRowC] = (double)f[12];
-    __syncthreads();
     const uint32_t te = tail ? T : ts + kSegSteps;
     const double nseg = (double)(te - ts);
+    if (h == 0 && is_c) {
+      SegVec& v = seg[row - kRowC];
+      const double sig = (double)f[12];
+      v.P = nseg * v.b + s * sig;
+      v.Q = s * sig;
+    }
+    __syncthreads();
     {
       // flush into the FP64 Gram.  Rows 24..26 (v = 12..14, lower half) hold
       // sum x, sum k x, sum k(k-1)/2 x: stored as sum x, sum u x, sum u^2 x with
@@ -453,10 +466,10 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
 #pragma unroll
     for (int q = 0; q < NG; ++q) {
       if (l + 64 * q < D * D) {
-        const double bi = seg_b0c[gi[q]], bj = seg_b0c[gj[q]];
-        const double si_ = seg_sig[gi[q]], sj = seg_sig[gj[q]];
-        bbA[q] += nseg * (bi * bj) + s * ((bi * sj) + (si_ * bj));
-        ebA[q] += (s * seg_eps[gi[q]]) * bj;
+        const SegVec& vi = seg[gi[q]];
+        const SegVec& vj = seg[gj[q]];
+        bbA[q] = fma(vi.b, vj.P, fma(vi.Q, vj.b, bbA[q]));
+        ebA[q] = fma(vi.E, vj.b, ebA[q]);
       }
     }
     if (l < 3 * D) {
@@ -471,7 +484,7 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
       const double s1 = (nseg * A + k1) * dt;
       const double s2 = ((nseg * A) * A + (2.0 * A) * k1 + k2) * dt2;
       const double up = mp == 0 ? nseg : mp == 1 ? s1 : s2;
-      mb[l] += seg_b0c[mi] * up;
+      mb[l] += seg[mi].b * up;
     }
     __syncthreads();
     // the next reads of b0s come two barriers into the next segment
