@@ -68,8 +68,9 @@ def cpu_baseline(model, dim, steps, threads, target_s=12.0):
 
     import oracle
 
-    # calibrate on a small sample, then time ~target_s of work
-    cal_n = 2 * threads
+    # calibrate on a small sample (16 walks per thread, so that thread start-up
+    # does not dominate it), then time ~target_s of work
+    cal_n = 16 * threads
     t0 = time.perf_counter()
     oracle.walks_by_seed_c(model, dim, steps, np.arange(1, cal_n + 1, dtype=np.uint32), threads=threads)
     rate = cal_n / (time.perf_counter() - t0)
