@@ -7,9 +7,12 @@
 // seed w+1 on key word `seed`, so a query's walks depend only on the query:
 // its result is the same alone and in any mix.
 //
-// Continuous batching: a scheduler thread keeps the admitted queries in a
-// round-robin ring and hands their walks out in chunks to `lanes` launch
-// lanes (each a HIP stream with its own engine context, default priority).
+// Continuous batching: a scheduler thread keeps the admitted queries and
+// hands their walks out in chunks to `lanes` launch lanes (each a HIP stream
+// with its own engine context, default priority), the next chunk of a free
+// lane going to the query with the least work left (walks x steps x dim:
+// shortest remaining work first, which minimises the mean latency; queries
+// with equal work go in arrival order).
 // A query that arrives while others run is admitted as soon as its HBM
 // footprint (2 doubles per walk for its per-walk statistics) fits the
 // budget, and its chunks interleave with the running queries' chunks on the
@@ -62,7 +65,19 @@ __global__ void walk_statistics(const double* __restrict__ eigs, int nf, uint32_
   maxeig[i] = m;
 }
 
+// A chunk is at most 65,536 walks and at most 2^31 normals (walks x steps x
+// dim; about 3 ms of the whole GPU at dim 12): long high-dim queries come in
+// smaller chunks, so a query that arrives behind them waits less for a lane.
 constexpr uint32_t kChunkWalks = 1u << 16;
+constexpr uint64_t kChunkNormals = 1ull << 31;
+constexpr uint32_t kMinChunkWalks = 4096;
+
+uint32_t chunk_walks(uint32_t steps, int dim) {
+  const uint64_t per_walk = (uint64_t)steps * (uint64_t)dim;
+  const uint64_t n = kChunkNormals / (per_walk ? per_walk : 1);
+  if (n >= kChunkWalks) return kChunkWalks;
+  return n <= kMinChunkWalks ? kMinChunkWalks : (uint32_t)(n & ~uint64_t(255));
+}
 
 struct Request {
   uint64_t id;
@@ -126,8 +141,7 @@ struct jne_server {
   std::condition_variable cv;       // scheduler wake-up
   std::condition_variable done_cv;  // completions
   std::deque<std::shared_ptr<Request>> pending;
-  std::vector<std::shared_ptr<Request>> active;  // admitted, round-robin
-  size_t rr = 0;
+  std::vector<std::shared_ptr<Request>> active;  // admitted, in arrival order
   std::map<uint64_t, std::shared_ptr<Request>> all;
   std::deque<std::shared_ptr<Request>> to_finish;
   uint64_t next_id = 1;
@@ -165,7 +179,7 @@ void fail_request(jne_server* s, const std::shared_ptr<Request>& r, int code, co
 // Issues the next chunk of request r on lane L (called with s->mu held).
 int issue(jne_server* s, Lane& L, const std::shared_ptr<Request>& r) {
   const uint32_t first = (uint32_t)(r->issued + 1);
-  const uint32_t n = (uint32_t)std::min<uint64_t>(kChunkWalks, r->runs - r->issued);
+  const uint32_t n = (uint32_t)std::min<uint64_t>(chunk_walks(r->steps, r->dim), r->runs - r->issued);
   hipError_t e = hipSetDevice(s->device);
   if (e == hipSuccess) {
     hipLaunchKernelGGL(iota_seeds, dim3((n + 255) / 256), dim3(256), 0, L.st, L.d_seeds, first, n);
@@ -231,15 +245,18 @@ void scheduler_loop(jne_server* s) {
       }
       release_if_drained(s, r);  // a query failed by another lane's chunk
     }
-    // hand out chunks round-robin over the admitted queries
+    // hand out chunks: each free lane takes the next chunk of the admitted
+    // query with the least work left
     for (Lane& L : s->lanes) {
       if (L.req) continue;
       std::shared_ptr<Request> pick;
-      for (size_t k = 0; k < s->active.size() && !pick; ++k) {
-        auto& r = s->active[(s->rr + k) % s->active.size()];
-        if (r->status == 0 && r->issued < r->runs) {
+      uint64_t least = UINT64_MAX;
+      for (const auto& r : s->active) {
+        if (r->status != 0 || r->issued >= r->runs) continue;
+        const uint64_t left = (r->runs - r->issued) * (uint64_t)r->steps * (uint64_t)r->dim;
+        if (left < least) {
+          least = left;
           pick = r;
-          s->rr = (s->rr + k + 1) % s->active.size();
         }
       }
       if (!pick) break;
