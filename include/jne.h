@@ -245,9 +245,10 @@ int jne_run_simulation(jne_ctx *ctx, int model, int dim, uint32_t steps, uint64_
  * precision): the same alone and in any mix of concurrent queries.
  * Queries are admitted in arrival order when their device footprint
  * (16 bytes per walk) fits the HBM budget; admitted queries share `lanes`
- * launch streams chunk by chunk (continuous batching).  The quantiles are
- * selected on the device (radix sort of the per-walk statistics); only the
- * six values come back to the host.
+ * launch streams chunk by chunk (continuous batching), a free stream taking
+ * the next chunk of the query with the least work left.  The quantiles are
+ * selected on the device (bucket selection of the per-walk statistics, as
+ * jne_percentiles_device); only the six values come back to the host.
  * ------------------------------------------------------------------------ */
 #define JNE_ERR_TIMEOUT (-8)
 typedef struct jne_server jne_server;
