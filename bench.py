@@ -63,7 +63,7 @@ def relative_floor(got, ref, floor=1e-5):
     return np.abs(got - ref) / den
 
 
-def cpu_baseline(model, dim, steps, threads, target_s=12.0):
+def cpu_baseline(model, dim, steps, threads, target_s=15.0):
     import numpy as np
 
     import oracle
