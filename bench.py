@@ -113,7 +113,7 @@ def serve_at_rate(server, reqs, rate, precision, rng):
     return lat, time.perf_counter() - t0
 
 
-def serving_sweep(pkg, rates, n_req, precision, seed=2024):
+def serving_sweep(pkg, rates, n_req, precision, seed=2024, lanes=4):
     import numpy as np
 
     rng = np.random.default_rng(seed)
@@ -123,7 +123,7 @@ def serving_sweep(pkg, rates, n_req, precision, seed=2024):
     out = {"precision": precision, "requests_per_rate": n_req,
            "mix": "model U{0..4}, dim U{1..12}, T log-U[100,1e4], runs log-U[1e4,1e6], own seed",
            "walks_per_rate": walks, "rates": []}
-    with pkg.Server(0, hbm_budget_bytes=8 << 30, lanes=4) as server:
+    with pkg.Server(0, hbm_budget_bytes=8 << 30, lanes=lanes) as server:
         server.wait(server.submit(4, 12, 1000, 70000, 1, precision=precision))  # warm the lanes
         for rate in rates:
             lat, wall = serve_at_rate(server, reqs, rate, precision, np.random.default_rng(seed + int(rate)))
@@ -342,6 +342,9 @@ def main() -> int:
                     help="Poisson query rates (req/s) for the serving sweep ('' to skip)")
     ap.add_argument("--serve-requests", type=int, default=100, help="queries per rate")
     ap.add_argument("--serve-precision", default="auto", help="precision of the served queries")
+    ap.add_argument("--serve-lanes", type=int, default=4, help="launch lanes of the query server")
+    ap.add_argument("--no-serve-fp64", action="store_true",
+                    help="skip the FP64 serving run at the highest rate (the saturation comparison)")
     ap.add_argument("--ladder", type=int, default=0,
                     help="print the precision ladder on this many walks instead of timing")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -451,11 +454,12 @@ def main() -> int:
 
     if args.serve_rates:
         rates = [float(r) for r in args.serve_rates.split(",")]
-        sv = serving_sweep(pkg, rates, args.serve_requests, args.serve_precision)
-        # the same query sequence on the FP64 path at the highest rate: both saturated
-        top = serving_sweep(pkg, [rates[-1]], args.serve_requests, "fp64")["rates"][0]
-        sv["fp64_at_top_rate"] = top
-        sv["saturated_mixed_vs_fp64"] = sv["rates"][-1]["walk_steps_per_s"] / top["walk_steps_per_s"]
+        sv = serving_sweep(pkg, rates, args.serve_requests, args.serve_precision, lanes=args.serve_lanes)
+        if not args.no_serve_fp64:
+            # the same query sequence on the FP64 path at the highest rate: both saturated
+            top = serving_sweep(pkg, [rates[-1]], args.serve_requests, "fp64", lanes=args.serve_lanes)["rates"][0]
+            sv["fp64_at_top_rate"] = top
+            sv["saturated_mixed_vs_fp64"] = sv["rates"][-1]["walk_steps_per_s"] / top["walk_steps_per_s"]
         line["serving"] = sv
 
     if not args.no_cpu_baseline:
