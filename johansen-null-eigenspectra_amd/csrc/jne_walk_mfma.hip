@@ -88,6 +88,13 @@ template <int D>
 struct MfmaShape {
   static constexpr int CALLS = kSegSteps * D / 4;       // Philox calls per segment
   static constexpr int IT = (CALLS + 63) / 64;          // call iterations per lane
+  // By-seed form, when a segment's calls are not whole 64-call iterations:
+  // the spare lanes of a segment's last iteration draw the next segment's
+  // first calls (the same Philox counters), which wait in an LDS buffer of
+  // at most OVF calls, so the segments draw CALLS calls each on average
+  // instead of 64 IT.
+  static constexpr bool CARRY = CALLS > 64 && CALLS % 64 != 0;
+  static constexpr int OVF = CARRY ? 64 - gcd_int(CALLS, 64) : 1;
   static constexpr int ZP = kSegSteps + 4;              // LDS pitch of one row (floats)
   static constexpr int NG = (D * D + 63) / 64;          // (i, j) grid entries per lane
   static constexpr int SP = D <= 8 ? 8 : 4;             // scan lanes per component
@@ -258,6 +265,9 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
   float(*const cbuf)[ZP] = xs + kRowC;
   double* const gram = reinterpret_cast<double*>(rows_raw);
   __shared__ double eps_part[SP][D];
+  // (not with the three-way bf16 split, whose operands leave no registers for it)
+  constexpr bool kCarry = Sh::CARRY && !FROM_NORMALS && PREC != JNE_PREC_MFMA_BF16X3;
+  __shared__ float4 ovf[kCarry ? Sh::OVF : 1];  // calls drawn ahead for the next segment
   __shared__ double seg_eps[D];
   __shared__ double b0s[D];  // B at the segment start (FP64)
   // per component j of a segment: b = B0 + s m (the state the centred c rows
@@ -336,6 +346,7 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
   const double s = uniform_f64(sqrt(dt));
   const double dt2 = uniform_f64(dt * dt);
   const uint32_t rounds = (T + kSegSteps - 1) / kSegSteps;
+  uint32_t drawn = 0;  // calls drawn so far (carry form): a multiple of 64
 
   // one segment; the partial last segment (T not a multiple of 64) runs a copy
   // with the step masks, so the loop over full segments carries none of them
@@ -343,6 +354,44 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
     constexpr bool tail = decltype(tail_tag)::value;
     const uint32_t ts = r * kSegSteps;
     // ---- normals of the segment -> LDS
+    if constexpr (kCarry) {
+      // calls [r CALLS, drawn) came with the previous segment's last iteration
+      const int pending = (int)(drawn - r * (uint32_t)Sh::CALLS);
+      if (l < pending) {
+        const float4 q = ovf[l];
+        const float qv[4] = {q.x, q.y, q.z, q.w};
+        const CallPos<D> p(l);
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+          int tl, i;
+          p.at(jj, tl, i);
+          zbuf[i][tl] = (!tail || ts + tl < T) ? qv[jj] : 0.0f;
+        }
+      }
+      // the rest, 64 calls per iteration; calls past the segment (spare lanes
+      // of the last iteration) wait in ovf for the next one
+      const uint32_t need = (r + 1) * (uint32_t)Sh::CALLS - drawn;
+      const int nit = (int)((need + 63) / 64);
+      int c = pending + l;  // this lane's call, counted from the segment's first
+      CallPos<D> pos(c);
+      for (int j = 0; j < nit; ++j) {
+        float v[4];
+        normals4((uint64_t)drawn + 64 * j + l, seed, rng_stream, v);
+        if (c < Sh::CALLS) {
+#pragma unroll
+          for (int jj = 0; jj < 4; ++jj) {
+            int tl, i;
+            pos.at(jj, tl, i);
+            zbuf[i][tl] = (!tail || ts + tl < T) ? v[jj] : 0.0f;
+          }
+        } else if (!tail) {
+          ovf[c - Sh::CALLS] = make_float4(v[0], v[1], v[2], v[3]);
+        }
+        c += 64;
+        pos.template advance<64>();
+      }
+      drawn += 64u * (uint32_t)nit;
+    } else {
     CallPos<D> pos = pos0;
 #pragma unroll CallUnroll<D, PREC, FROM_NORMALS>::value
     for (int j = 0; j < Sh::IT; ++j) {
@@ -363,6 +412,7 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
         zbuf[i][tl] = z;
       }
       pos.template advance<64>();
+    }
     }
     __syncthreads();
 
