@@ -28,7 +28,9 @@
 // drive the state B0 are formed in FP64 from the stored normals.
 //
 // One wave per walk.  A segment runs in three phases: the wave draws its
-// normals into LDS rows; a scan lays the c rows beside them (lane (i, p)
+// normals into LDS rows (64 Philox calls per iteration; where a segment's
+// calls are not whole iterations, the last one's spare lanes draw the next
+// segment's first calls); a scan lays the c rows beside them (lane (i, p)
 // takes a block of steps of component i, sums it in FP64 for eps and writes
 // each c_t as the FP32 rounding of an FP64 running sum); then lane l feeds
 // row l%32 of X for 8 of every 16 steps (half l/32), read straight from its
