@@ -148,6 +148,7 @@ struct jne_server {
   uint64_t epoch = 0;  // bumped by submissions and freed memory: wakes an idle scheduler
   bool stop = false;
   std::thread scheduler, finisher;
+  hipStream_t fin_st = nullptr;  // the finisher's stream, created before the lanes' streams
   // counters
   uint64_t walks_done = 0;
   double busy_s = 0.0;
@@ -288,9 +289,8 @@ void scheduler_loop(jne_server* s) {
 }
 
 void finisher_loop(jne_server* s) {
-  hipStream_t st = nullptr;
+  hipStream_t st = s->fin_st;
   (void)hipSetDevice(s->device);
-  (void)hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
   {
     jne::QuantileScratch sc;
     std::unique_lock<std::mutex> lk(s->mu);
@@ -326,7 +326,6 @@ void finisher_loop(jne_server* s) {
       s->cv.notify_all();  // memory freed: the scheduler may admit more
     }
   }
-  if (st) (void)hipStreamDestroy(st);
 }
 
 }  // namespace
@@ -342,9 +341,24 @@ int jne_server_create(int device, uint64_t hbm_budget_bytes, int lanes, jne_serv
   jne_server* s = new jne_server();
   s->device = device;
   hipError_t e = hipSetDevice(device);
+  // The finisher's stream: HIP maps ordinary streams onto a few hardware
+  // queues (4 by default), each running its work in order, so a finisher
+  // stream that shares a queue with a lane waits behind that lane's chunks
+  // at every selection.  A stream with a CU mask (here: every CU) gets a
+  // queue of its own; an ordinary stream, created before the lanes', is the
+  // fallback.
+  if (e == hipSuccess) {
+    std::vector<uint32_t> all_cus(16, 0xFFFFFFFFu);  // up to 512 CUs; extra bits are ignored
+    if (hipExtStreamCreateWithCUMask(&s->fin_st, (uint32_t)all_cus.size(), all_cus.data()) != hipSuccess) {
+      (void)hipGetLastError();
+      s->fin_st = nullptr;
+      e = hipStreamCreateWithFlags(&s->fin_st, hipStreamNonBlocking);
+    }
+  }
   if (e == hipSuccess) e = hipMalloc((void**)&s->arena_base, hbm_budget_bytes);
   if (e != hipSuccess) {
     jne_host::set_error(std::string("jne_server_create: ") + hipGetErrorString(e));
+    if (s->fin_st) (void)hipStreamDestroy(s->fin_st);
     delete s;
     return JNE_ERR_NOMEM;
   }
@@ -392,6 +406,7 @@ void jne_server_destroy(jne_server* s) {
     if (L.ctx) jne_destroy(L.ctx);
   }
   if (s->arena_base) (void)hipFree(s->arena_base);
+  if (s->fin_st) (void)hipStreamDestroy(s->fin_st);
   delete s;
 }
 
