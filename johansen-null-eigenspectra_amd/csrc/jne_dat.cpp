@@ -1,5 +1,5 @@
 // jne_dat.cpp -- the append-only .dat V6 store, its reader, resume and
-// percentiles, plus the display helpers the driver and the CLI print with.
+// percentiles, plus the display helpers the simulation driver and the CLI print with.
 //
 // Byte format (DATA_FORMAT.md, src/data_storage/file_format.rs:8-9,
 // writer.rs:61-64,241-249,262-289):
