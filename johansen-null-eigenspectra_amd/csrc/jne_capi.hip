@@ -190,11 +190,12 @@ hipError_t timed_launch(jne_ctx* ctx, int kernel, uint64_t walks, hipStream_t st
 
 // Enqueue the walk path for n walks (device pointers) on stream st.
 // JNE_PREC_AUTO: the MFMA pass where it measured faster than the FP64 pass
-// (walks/s, model 4, T = 100..10^4): dims 10-12 from 300 steps and dim 9 from
-// 500; short walks carry the MFMA pass's per-walk setup.
+// (walks/s, model 4, T = 100..10^4): dims 10-12 from 300 steps, dim 9 from 500
+// and dim 8 from 3,000; short walks carry the MFMA pass's per-walk setup.
 bool auto_uses_mfma(int dim, uint32_t steps) {
   if (dim >= 10) return steps >= 300;
   if (dim == 9) return steps >= 500;
+  if (dim == 8) return steps >= 3000;
   return false;
 }
 
