@@ -63,6 +63,39 @@ def relative_floor(got, ref, floor=1e-5):
     return np.abs(got - ref) / den
 
 
+def host_cpu_info():
+    """The host's CPU model (lscpu "Model name"), logical CPUs (nproc), the CPUs
+    this process may run on (affinity), its cgroup CPU quota and OMP_NUM_THREADS
+    (the job's CPU share where the host is shared), and the threads the CPU
+    baseline uses: every CPU of that share."""
+    import subprocess
+
+    info = {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        info["model_name"] = next((l.split(":", 1)[1].strip() for l in out.splitlines()
+                                   if l.startswith("Model name")), None)
+    except (OSError, subprocess.SubprocessError):
+        info["model_name"] = None
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = int(q) / int(period)
+    except (OSError, ValueError):
+        pass
+    info["cgroup_cpu_quota"] = quota
+    omp = os.environ.get("OMP_NUM_THREADS")
+    info["omp_num_threads"] = int(omp) if omp and omp.isdigit() else None
+    share = info["affinity_cpus"]
+    if quota:
+        share = min(share, max(1, int(quota)))
+    if info["omp_num_threads"]:
+        share = min(share, info["omp_num_threads"])
+    info["cpu_share"] = share
+    return info
+
+
 def cpu_baseline(model, dim, steps, threads, target_s=15.0):
     import numpy as np
 
@@ -113,6 +146,25 @@ def serve_at_rate(server, reqs, rate, precision, rng):
     return lat, time.perf_counter() - t0
 
 
+# query size classes by the normals a query draws (runs x T x dim): the
+# largest class holds the 10^6-walk, dim-12, long queries (up to 1.2e11)
+SIZE_CLASSES = [("normals<1e8", 0.0, 1e8), ("1e8<=normals<1e9", 1e8, 1e9), ("1e9<=normals<1e10", 1e9, 1e10),
+                ("normals>=1e10", 1e10, float("inf"))]
+
+
+def latency_by_class(reqs, lat):
+    import numpy as np
+
+    size = np.array([r[1] * r[2] * r[3] for r in reqs], dtype=np.float64)
+    out = {}
+    for name, lo, hi in SIZE_CLASSES:
+        sel = lat[(size >= lo) & (size < hi)]
+        out[name] = {"queries": int(len(sel)),
+                     "p50_ms": float(np.percentile(sel, 50)) if len(sel) else None,
+                     "p99_ms": float(np.percentile(sel, 99)) if len(sel) else None}
+    return out
+
+
 def serving_sweep(pkg, rates, n_req, precision, seed=2024, lanes=4):
     import numpy as np
 
@@ -130,7 +182,10 @@ def serving_sweep(pkg, rates, n_req, precision, seed=2024, lanes=4):
             out["rates"].append({"offered_req_per_s": rate, "p50_ms": float(np.percentile(lat, 50)),
                                  "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
                                  "walks_per_s": walks / wall, "walk_steps_per_s": walk_steps / wall,
-                                 "achieved_req_per_s": len(reqs) / wall})
+                                 "achieved_req_per_s": len(reqs) / wall,
+                                 "by_size_class": latency_by_class(reqs, lat)})
+            print(f"bench: served {len(reqs)} queries at {rate:g} req/s ({precision}) in {wall:.1f} s",
+                  file=sys.stderr, flush=True)
     return out
 
 
@@ -250,11 +305,39 @@ def rng_rate(timeout_s=120):
     return json.loads(res.stdout.strip().splitlines()[-1])
 
 
+def valu_issue_for(key):
+    """VALU issue of this moment kernel and of the generator alone
+    (bin/jne_rng_rate) from the committed counter runs (profiles/valu_busy.json:
+    rocprofv3 SQ_ACTIVE_INST_VALU, SQ_BUSY_CU_CYCLES): VALU wave instructions
+    issued per CU-cycle and per normal drawn.  frac_of_rng factors into
+    issue_rate_vs_rng (how fast the kernel issues against the generator alone)
+    times insts_vs_rng (the generator's instructions per normal over the
+    kernel's: the share of the kernel's issue that is the normals)."""
+    path = os.path.join(ROOT, "profiles", "valu_busy.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f)
+    k, g = rec.get(key), rec.get("rng_rate_kernel")
+    if not k or not g:
+        return None
+    return {"kernel_valu_insts_per_cu_cycle": k["valu_insts_per_cu_cycle"],
+            "rng_valu_insts_per_cu_cycle": g["valu_insts_per_cu_cycle"],
+            "kernel_valu_insts_per_normal": k["valu_insts_per_normal"],
+            "rng_valu_insts_per_normal": g["valu_insts_per_normal"],
+            "issue_rate_vs_rng": k["valu_insts_per_cu_cycle"] / g["valu_insts_per_cu_cycle"],
+            "insts_vs_rng": g["valu_insts_per_normal"] / k["valu_insts_per_normal"],
+            "counter_frac_of_rng": (k["valu_insts_per_cu_cycle"] / g["valu_insts_per_cu_cycle"])
+            * (g["valu_insts_per_normal"] / k["valu_insts_per_normal"]),
+            "source": "rocprofv3 --pmc SQ_ACTIVE_INST_VALU SQ_BUSY_CU_CYCLES (profiles/valu_busy.json, "
+                      "commands in profiles/valu_busy.md)"}
+
+
 def kernel_name(precision: str, dim: int) -> str:
     if precision == "fp64":
         return f"walk_moments_fp64<{dim}>"
     if precision == "fp32_kahan":
-        return f"walk_moments_fp64<{dim}, kahan>"
+        return f"walk_moments_mfma<{dim}, fp32_kahan>"
     return f"walk_moments_mfma<{dim}, {precision}>"
 
 
@@ -279,11 +362,18 @@ def moment_roofline(precision, model, dim, T, mom, wps):
         if rec:
             traffic = (rec["fetch_bytes_per_walk"] + rec["write_bytes_per_walk"]) * walks_per_launch
     roofline = {
-        "bound": "mfma",
+        # both moment passes are bound by VALU instruction issue (the normal
+        # generator is most of it); achieved/peak/frac are the algorithmic
+        # contraction against the peak of the unit the Gram runs on, and
+        # valu_issue gives the counter-measured VALU busy fractions
+        "bound": "valu",
         "achieved": achieved,
         "peak": peak,
         "unit": "TFLOP/s",
         "frac": achieved / peak,
+        "gram_unit": "mfma_f16" if precision.startswith("mfma") and precision != "mfma_fp32" else
+                     ("mfma_f32" if precision == "mfma_fp32" else ("fp32_valu" if precision == "fp32_kahan"
+                                                                   else "fp64_valu")),
         "traffic": traffic,
         "traffic_unit": "bytes per launch (HBM, rocprofv3 FETCH_SIZE + WRITE_SIZE)",
         "algorithmic_bytes_per_launch": (4 + 8 * (2 * dim * dim + 6 * dim)) * walks_per_launch,
@@ -293,6 +383,7 @@ def moment_roofline(precision, model, dim, T, mom, wps):
         "walks_per_launch": walks_per_launch,
         "normals_per_s": wps * T * dim,
         "unit_rates": unit_rates_for(key, walks_per_launch, avg_ms),
+        "valu_issue": valu_issue_for(key),
     }
     if precision in MFMA_PER_CHUNK:
         issued = MFMA_PER_CHUNK[precision] * 2 * 32 * 32 * 16 * ((T + 15) // 16)
@@ -300,7 +391,7 @@ def moment_roofline(precision, model, dim, T, mom, wps):
     return roofline
 
 
-def cpu_baseline_block(model, dim, T, threads, head, fp64_head, k):
+def cpu_baseline_block(model, dim, T, threads, head, fp64_head, k, host):
     """cpu_baseline object: the C port on `threads` host threads over a bounded
     sample, and its agreement with the GPU on the first walks."""
     import numpy as np
@@ -312,7 +403,9 @@ def cpu_baseline_block(model, dim, T, threads, head, fp64_head, k):
         "cores": threads,
         "kind": "port",
         "sample": f"{cn:,} walks (seeds 1..{cn:,}) of the same workload in {cdt:.1f} s; C restatement of "
-                  f"the reference CPU walk (oracle/jne_oracle.c), one walk per thread",
+                  f"the reference CPU walk (oracle/jne_oracle.c), one walk per thread, on every CPU of "
+                  f"this job's share of the host",
+        "host": host,
     }
     m = min(64, cn, k)
     if fp64_head is not None:
@@ -321,6 +414,44 @@ def cpu_baseline_block(model, dim, T, threads, head, fp64_head, k):
             (d / (np.abs(ceig[:m]) + 1e-13 * ceig[:m].max(axis=1, keepdims=True))).max())
     block["headline_vs_cpu_max_rel_diff_first_64"] = float(relative_floor(head[:m], ceig[:m]).max())
     return block
+
+
+def config1_full(pkg, eng, threads, host, model=0, dim=2, T=1000, runs=100_000):
+    """The reference's CPU-native configuration in full (model 0, dim 2,
+    T = 1,000, 100,000 runs; src/data_storage/parallel_compute.rs:14-41): the
+    C port on the CPU share, and on the GPU both the batch path (seeds in,
+    eigenvalues out, host buffers) and the streamed run that writes
+    eigenvalues_model0_dim2_steps1000.dat (jne_run_simulation, from an empty
+    directory, end to end); the GPU and CPU results are compared walk by walk."""
+    import tempfile
+
+    import numpy as np
+
+    import oracle
+
+    seeds = np.arange(1, runs + 1, dtype=np.uint32)
+    t0 = time.perf_counter()
+    cpu = oracle.walks_by_seed_c(model, dim, T, seeds, threads=threads)
+    cpu_s = time.perf_counter() - t0
+    eng.eigenvalues_by_seed(model, dim, T, seeds[:1000])  # warm the kernels and the buffers
+    t0 = time.perf_counter()
+    gpu = eng.eigenvalues_by_seed(model, dim, T, seeds)
+    gpu_s = time.perf_counter() - t0
+    with tempfile.TemporaryDirectory() as tmp:
+        sim = pkg.EigenvalueSimulation(model, dim, T, runs, engine=eng, data_dir=tmp)
+        t0 = time.perf_counter()
+        sim.run_simulation_quiet()
+        sim_s = time.perf_counter() - t0
+        path = sim.get_filename()
+        dat_bytes = os.path.getsize(path)
+        fs, fe, *_ = pkg.read_dat(path)
+        same = bool(np.array_equal(fe, gpu[fs.astype(np.int64) - 1]))
+    rel = np.abs(gpu - cpu) / (np.abs(cpu) + 1e-13 * cpu.max(axis=1, keepdims=True))
+    return {"workload": f"Johansen model {model}, dim {dim}, T={T:,} steps, {runs:,} runs (seeds 1..{runs:,}), "
+                        f"FP64 path", "cpu_port_walks_per_s": runs / cpu_s, "cpu_s": cpu_s, "cpu_threads": threads,
+            "cpu_host": host.get("model_name"), "gpu_batch_walks_per_s": runs / gpu_s, "gpu_batch_s": gpu_s,
+            "gpu_streamed_dat_s": sim_s, "gpu_streamed_walks_per_s": runs / sim_s, "dat_file_bytes": dat_bytes,
+            "dat_equals_batch": same, "gpu_vs_cpu_max_rel_diff": float(rel.max())}
 
 
 def main() -> int:
@@ -337,10 +468,13 @@ def main() -> int:
     ap.add_argument("--check-walks", type=int, default=10_000, help="walks compared between the two paths")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-rng-rate", action="store_true", help="skip the generator-alone rate (bin/jne_rng_rate)")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cores)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="0 = every CPU of this job's share of the host (affinity, cgroup quota, OMP_NUM_THREADS)")
+    ap.add_argument("--no-config1", action="store_true",
+                    help="skip the plain-CPU configuration (model 0, dim 2, T=1,000, 100,000 runs) on CPU and GPU")
     ap.add_argument("--serve-rates", default="10,30,100,300,1000",
                     help="Poisson query rates (req/s) for the serving sweep ('' to skip)")
-    ap.add_argument("--serve-requests", type=int, default=100, help="queries per rate")
+    ap.add_argument("--serve-requests", type=int, default=1000, help="queries per rate")
     ap.add_argument("--serve-precision", default="auto", help="precision of the served queries")
     ap.add_argument("--serve-lanes", type=int, default=4, help="launch lanes of the query server")
     ap.add_argument("--no-serve-fp64", action="store_true",
@@ -442,6 +576,7 @@ def main() -> int:
             "moments_avg_launch_ms": c_avg,
             "unit_rates": unit_rates_for(f"walk_moments_{args.compare}_m{model}_d{dim}_T{T}",
                                          cmom[2] / max(cmom[1], 1), c_avg),
+            "valu_issue": valu_issue_for(f"walk_moments_{args.compare}_m{model}_d{dim}_T{T}"),
             "frac_of_rng": cwps * T * dim / rng["normals_per_s"] if rng else None,
             "speedup": wps / cwps,
             "max_rel_diff": float(rel.max()),
@@ -462,9 +597,12 @@ def main() -> int:
             sv["saturated_mixed_vs_fp64"] = sv["rates"][-1]["walk_steps_per_s"] / top["walk_steps_per_s"]
         line["serving"] = sv
 
+    host = host_cpu_info()
+    threads = args.cpu_threads or host["cpu_share"]
+    if not args.no_config1:
+        line["config1"] = config1_full(pkg, eng, threads, host)
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        line["cpu_baseline"] = cpu_baseline_block(model, dim, T, threads, head, fp64_head, k)
+        line["cpu_baseline"] = cpu_baseline_block(model, dim, T, threads, head, fp64_head, k, host)
     print(json.dumps(line), flush=True)
     eng.close()
     return 0

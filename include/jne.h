@@ -87,12 +87,18 @@ int jne_num_eigenvalues(int model, int dim);
 int jne_eigenvalues_by_seed(jne_ctx *ctx, int model, int dim, uint32_t steps, uint32_t rng_stream,
                             const uint32_t *seeds, size_t n, int precision, double *out);
 
+/* Stream arguments of the device entry points: a hipStream_t.  NULL is HIP's
+ * null (default) stream, which HIP orders with every blocking stream of the
+ * device, as in any HIP call; JNE_STREAM_CONTEXT selects the context's own
+ * (non-blocking) stream. */
+#define JNE_STREAM_CONTEXT ((void *)(intptr_t)-1)
+
 /* Same with device pointers (d_seeds, d_out in device memory of ctx's device);
- * the work is enqueued on `hip_stream` (a hipStream_t; NULL = the context's
- * stream) and the call returns without synchronising.  Non-positive-definite
- * walks yield NaN rows; no host-side check is made.  The context's device
- * scratch is shared by its calls: a call on another stream than the previous
- * call first waits (on the device) for the previous call's use of it. */
+ * the work is enqueued on `hip_stream` (see JNE_STREAM_CONTEXT) and the call
+ * returns without synchronising.  Non-positive-definite walks yield NaN rows;
+ * no host-side check is made.  The context's device scratch is shared by its
+ * calls: a call on another stream than the previous call first waits (on the
+ * device) for the previous call's use of it. */
 int jne_eigenvalues_by_seed_device(jne_ctx *ctx, int model, int dim, uint32_t steps, uint32_t rng_stream,
                                    const uint32_t *d_seeds, size_t n, int precision, double *d_out,
                                    void *hip_stream);
@@ -113,8 +119,8 @@ int jne_walk_normals(jne_ctx *ctx, int dim, uint32_t steps, uint32_t rng_stream,
                      float *out);
 
 /* Same with device pointers (d_seeds: n seeds, d_out: n * steps * dim floats
- * in device memory of ctx's device), enqueued on `hip_stream` (NULL = the
- * context's stream) without synchronising; steps * dim < 2^32.  For callers
+ * in device memory of ctx's device), enqueued on `hip_stream` (see
+ * JNE_STREAM_CONTEXT) without synchronising; steps * dim < 2^32.  For callers
  * that consume the stream on the device.  Also replaces gen_normal_matrix
  * (src/rng_matrix.rs:11-37). */
 int jne_walk_normals_device(jne_ctx *ctx, int dim, uint32_t steps, uint32_t rng_stream, const uint32_t *d_seeds,
@@ -125,7 +131,7 @@ int jne_walk_normals_device(jne_ctx *ctx, int dim, uint32_t steps, uint32_t rng_
  * linear interpolation between the neighbouring order statistics, the same
  * values as jne_percentiles, found on the device by bucket selection (a sort
  * only when ties crowd the selected buckets).  Synchronous on hip_stream
- * (NULL = the context's stream); n = 0 gives NaN as jne_percentiles.  The
+ * (see JNE_STREAM_CONTEXT); n = 0 gives NaN as jne_percentiles.  The
  * device counterpart of get_percentile_value (src/simulation_analyzers.rs:4-18),
  * used by the query server; NaN values are JNE_ERR_NUMERIC. */
 int jne_percentiles_device(jne_ctx *ctx, const double *d_values, uint64_t n, const double *p, size_t np,

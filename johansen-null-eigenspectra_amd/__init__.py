@@ -75,6 +75,15 @@ KERNEL_MOMENTS = 0
 KERNEL_EIGEN = 1
 KERNEL_NORMALS = 2
 
+# JNE_STREAM_CONTEXT: the context's own stream (a stream handle of 0 is HIP's
+# null stream, as in any HIP call)
+STREAM_CONTEXT = ctypes.c_void_p(-1).value
+
+
+def _stream_arg(hip_stream):
+    """None: the context's own stream; an int: that hipStream_t (0 = HIP's null stream)."""
+    return ctypes.c_void_p(STREAM_CONTEXT if hip_stream is None else int(hip_stream))
+
 _lib = None
 _lib_lock = threading.Lock()
 
@@ -297,11 +306,12 @@ class Engine:
         return out
 
     def eigenvalues_by_seed_device(self, model: int, dim: int, steps: int, d_seeds: int, n: int, d_out: int,
-                                   rng_stream: int = 0, precision: int = PREC_FP64, hip_stream: int = 0):
-        """Device pointers (ints); enqueues on hip_stream (0 = the context's stream)."""
+                                   rng_stream: int = 0, precision: int = PREC_FP64, hip_stream=None):
+        """Device pointers (ints); enqueues on hip_stream (None: the context's stream; 0: HIP's null
+        stream, e.g. torch's default stream) without synchronising."""
         self._check(self._lib.jne_eigenvalues_by_seed_device(
             self._ctx, int(model), int(dim), int(steps), int(rng_stream), ctypes.c_void_p(d_seeds), int(n),
-            int(precision), ctypes.c_void_p(d_out), ctypes.c_void_p(hip_stream or None)))
+            int(precision), ctypes.c_void_p(d_out), _stream_arg(hip_stream)))
 
     def eigenvalues_from_normals(self, model: int, z, precision: int = PREC_FP64) -> np.ndarray:
         """z: (n, steps, dim) float64 N(0,1) draws."""
@@ -323,20 +333,20 @@ class Engine:
         return out
 
     def walk_normals_device(self, dim: int, steps: int, d_seeds: int, n: int, d_out: int, rng_stream: int = 0,
-                            hip_stream: int = 0):
-        """Device pointers (ints): d_out receives n*steps*dim floats; enqueues on hip_stream (0 = the
-        context's stream)."""
+                            hip_stream=None):
+        """Device pointers (ints): d_out receives n*steps*dim floats; enqueues on hip_stream (None: the
+        context's stream; 0: HIP's null stream)."""
         self._check(self._lib.jne_walk_normals_device(
             self._ctx, int(dim), int(steps), int(rng_stream), ctypes.c_void_p(d_seeds), int(n),
-            ctypes.c_void_p(d_out), ctypes.c_void_p(hip_stream or None)))
+            ctypes.c_void_p(d_out), _stream_arg(hip_stream)))
 
-    def percentiles_device(self, d_values: int, n: int, ps, hip_stream: int = 0) -> list:
+    def percentiles_device(self, d_values: int, n: int, ps, hip_stream=None) -> list:
         """Percentiles ps (1..8 values in [0, 1]) of the device array d_values[0..n) (a device pointer,
         int), selected on the device; the rule and values of percentiles()."""
         p = np.ascontiguousarray(ps, dtype=np.float64)
         out = np.empty(max(len(p), 1), dtype=np.float64)
         self._check(self._lib.jne_percentiles_device(self._ctx, ctypes.c_void_p(d_values), int(n), p.ctypes.data,
-                                                     len(p), out.ctypes.data, ctypes.c_void_p(hip_stream or None)))
+                                                     len(p), out.ctypes.data, _stream_arg(hip_stream)))
         return [float(x) for x in out[:len(p)]]
 
     def synchronize(self):
@@ -522,18 +532,21 @@ class EigenvalueSimulation:
         seeds, eigs, *_ = read_dat(self.get_filename(self.model))
         return [(int(s), e.tolist()) for s, e in zip(seeds, eigs)]
 
+    def _count_mismatch(self, n: int):
+        """The error of read_data when the file holds n != num_runs records with seed <= num_runs."""
+        return JneError(JNE_ERR_FORMAT,
+                        f"Data count mismatch: expected {self.num_runs} records, found {n} records. "
+                        f"This indicates incomplete simulation data.\n"
+                        f"Solutions:\n"
+                        f"1. Use `read_all_data()` to read available data ({n} records)\n"
+                        f"2. Run `run_simulation()` to complete the simulation\n"
+                        f"3. Create a new EigenvalueSimulation with num_runs={n} to match existing data")
+
     def read_data(self):
         """Records with seed <= num_runs; an error unless there are exactly num_runs (simulation.rs:55-89)."""
         data = [r for r in self.read_all_data() if r[0] <= self.num_runs]
         if len(data) != self.num_runs:
-            n = len(data)
-            raise JneError(JNE_ERR_FORMAT,
-                           f"Data count mismatch: expected {self.num_runs} records, found {n} records. "
-                           f"This indicates incomplete simulation data.\n"
-                           f"Solutions:\n"
-                           f"1. Use `read_all_data()` to read available data ({n} records)\n"
-                           f"2. Run `run_simulation()` to complete the simulation\n"
-                           f"3. Create a new EigenvalueSimulation with num_runs={n} to match existing data")
+            raise self._count_mismatch(len(data))
         return data
 
     def get_filename(self, model=None) -> str:
@@ -543,10 +556,22 @@ class EigenvalueSimulation:
         return os.path.join(self.data_dir, f"eigenvalues_model{m}_dim{self.dim}_steps{self.steps}.dat")
 
     def _stat_percentiles(self, stat: int, ps):
-        data = self.read_data()
-        if not data:
+        """calculate_percentiles (simulation_analyzers.rs:42-64): the records read_data keeps, each
+        reduced by the statistic in C (jne_dat_statistics: a sequential sum, or a maximum folded from
+        f64::MIN, as the reference's aggregators), then the percentile rule -- without a Python list
+        of every record."""
+        L = load_library()
+        path = _path(self.get_filename(self.model))
+        if self.num_runs == 0:  # read_data keeps no record (seeds start at 1): an empty result
+            dat_info(self.get_filename(self.model))  # ... but a missing file is still an error
             return []
-        vals = [sum(e) if stat == STAT_TRACE else max(e, default=float("-inf")) for _, e in data]
+        n = ctypes.c_uint64()
+        _host_check(L.jne_dat_statistics(path, self.num_runs, int(stat), None, 0, ctypes.byref(n)))
+        if n.value != self.num_runs:
+            raise self._count_mismatch(n.value)
+        vals = np.empty(n.value, dtype=np.float64)
+        _host_check(L.jne_dat_statistics(path, self.num_runs, int(stat), vals.ctypes.data, n.value,
+                                         ctypes.byref(n)))
         return percentiles(vals, ps)
 
     def calculate_trace_percentiles(self, ps):

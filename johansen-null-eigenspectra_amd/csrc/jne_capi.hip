@@ -233,6 +233,13 @@ int run_walks(jne_ctx* ctx, int model, int dim, uint32_t steps, uint32_t rng_str
   return JNE_OK;
 }
 
+// The stream a device entry point enqueues on: JNE_STREAM_CONTEXT is the
+// context's own stream, anything else (NULL included: HIP's null stream) is
+// the caller's.
+hipStream_t stream_of(jne_ctx* ctx, void* hip_stream) {
+  return hip_stream == JNE_STREAM_CONTEXT ? ctx->stream : (hipStream_t)hip_stream;
+}
+
 int finish_host_out(jne_ctx* ctx, double* out, size_t count) {
   JNE_HIP_CHECK(ctx, hipMemcpyAsync(out, ctx->d_out, count * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   JNE_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
@@ -326,7 +333,7 @@ int jne_eigenvalues_by_seed_device(jne_ctx* ctx, int model, int dim, uint32_t st
   if (n == 0) return JNE_OK;
   if (!d_seeds || !d_out) return set_err(ctx, JNE_ERR_INVALID_ARG, "d_seeds/d_out must not be NULL");
   JNE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
-  hipStream_t st = hip_stream ? (hipStream_t)hip_stream : ctx->stream;
+  hipStream_t st = stream_of(ctx, hip_stream);
   return run_walks(ctx, model, dim, steps, rng_stream, d_seeds, nullptr, n, precision, d_out, st);
 }
 
@@ -395,7 +402,7 @@ int jne_walk_normals_device(jne_ctx* ctx, int dim, uint32_t steps, uint32_t rng_
   if (n == 0) return JNE_OK;
   if (!d_seeds || !d_out) return set_err(ctx, JNE_ERR_INVALID_ARG, "d_seeds/d_out must not be NULL");
   JNE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
-  hipStream_t st = hip_stream ? (hipStream_t)hip_stream : ctx->stream;
+  hipStream_t st = stream_of(ctx, hip_stream);
   JNE_HIP_CHECK(ctx, timed_launch(ctx, JNE_KERNEL_NORMALS, n, st, [&] {
                   return jne::launch_walk_normals(dim, d_seeds, rng_stream, steps, (uint32_t)n, d_out, st);
                 }));
@@ -418,7 +425,7 @@ int jne_percentiles_device(jne_ctx* ctx, const double* d_values, uint64_t n, con
   }
   if (!d_values) return set_err(ctx, JNE_ERR_INVALID_ARG, "d_values must not be NULL");
   JNE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
-  hipStream_t st = hip_stream ? (hipStream_t)hip_stream : ctx->stream;
+  hipStream_t st = stream_of(ctx, hip_stream);
   const int rc = jne::quantile_select(ctx->qs, &d_values, 1, n, p, (int)np, out, st, nullptr);
   if (rc == JNE_ERR_NUMERIC) return set_err(ctx, rc, "a value is NaN");
   if (rc) return set_err(ctx, rc, "percentile selection failed");

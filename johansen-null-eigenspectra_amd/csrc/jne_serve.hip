@@ -193,7 +193,12 @@ int issue(jne_server* s, Lane& L, const std::shared_ptr<Request>& r) {
   double* trace = reinterpret_cast<double*>(s->arena_base + r->arena_off) + r->issued;
   double* maxeig = reinterpret_cast<double*>(s->arena_base + r->arena_off) + r->runs + r->issued;
   hipLaunchKernelGGL(walk_statistics, dim3((n + 255) / 256), dim3(256), 0, L.st, L.d_eigs, r->nf, n, trace, maxeig);
-  if ((e = hipGetLastError()) != hipSuccess || (e = hipEventRecord(L.ev, L.st)) != hipSuccess) return JNE_ERR_HIP;
+  if ((e = hipGetLastError()) != hipSuccess || (e = hipEventRecord(L.ev, L.st)) != hipSuccess) {
+    // walk_statistics may be enqueued and write into the query's statistics:
+    // drain the lane before the caller fails the query and frees that memory
+    (void)hipStreamSynchronize(L.st);
+    return JNE_ERR_HIP;
+  }
   r->issued += n;
   ++r->inflight;
   L.req = r;

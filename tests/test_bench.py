@@ -40,6 +40,27 @@ def test_unit_rates_from_the_committed_counters():
     assert bench.unit_rates_for("walk_moments_none", 1.0, 1.0) is None
 
 
+def test_host_cpu_info_names_the_share_it_uses(monkeypatch):
+    monkeypatch.setenv("OMP_NUM_THREADS", "3")
+    h = bench.host_cpu_info()
+    assert h["nproc"] == os.cpu_count() and h["affinity_cpus"] >= 1
+    assert h["omp_num_threads"] == 3 and 1 <= h["cpu_share"] <= 3
+    monkeypatch.delenv("OMP_NUM_THREADS")
+    assert bench.host_cpu_info()["cpu_share"] <= len(os.sched_getaffinity(0))
+
+
+def test_latency_by_size_class():
+    # (model, dim, T, runs, seed): normals = dim * T * runs
+    reqs = [(0, 1, 100, 10_000, 1), (4, 12, 10_000, 1_000_000, 2), (4, 12, 10_000, 900_000, 3),
+            (2, 5, 1000, 100_000, 4)]
+    lat = np.array([1.0, 300.0, 200.0, 10.0])
+    c = bench.latency_by_class(reqs, lat)
+    assert c["normals<1e8"] == {"queries": 1, "p50_ms": 1.0, "p99_ms": 1.0}
+    assert c["normals>=1e10"]["queries"] == 2 and c["normals>=1e10"]["p50_ms"] == 250.0
+    assert c["1e8<=normals<1e9"]["queries"] == 1 and c["1e9<=normals<1e10"]["queries"] == 0
+    assert c["1e9<=normals<1e10"]["p99_ms"] is None
+
+
 def test_algorithmic_flops():
     # 2 T nF (nF + d): the reference's S11 and S01 contractions
     assert bench.algorithmic_flops_per_walk(4, 12, 10000) == 2 * 10000 * 12 * 24

@@ -125,6 +125,28 @@ def test_device_calls_on_two_streams(engine, jne):
         assert np.array_equal(got[k], ref), k
 
 
+def test_device_call_on_torchs_default_stream(engine, jne):
+    """A device call on torch's default stream (handle 0: HIP's null stream)
+    is ordered after the torch work that produced its inputs and before the
+    torch copy that reads its outputs, with no device synchronisation in
+    between."""
+    import torch
+
+    model, dim, steps, n = 4, 12, 2000, 50_000
+    dev = torch.device("cuda:0")
+    for _ in range(2):
+        seeds = torch.arange(1, n + 1, dtype=torch.int32, device=dev) * 3  # enqueued on the null stream
+        out = torch.full((n, dim), float("nan"), dtype=torch.float64, device=dev)
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        assert stream == 0
+        engine.eigenvalues_by_seed_device(model, dim, steps, seeds.data_ptr(), n, out.data_ptr(),
+                                          precision=jne.PRECISIONS["mfma_fp16x2"], hip_stream=stream)
+        got = out.cpu().numpy()  # a null-stream copy: waits for the walk kernels
+        ref = engine.eigenvalues_by_seed(model, dim, steps, np.arange(1, n + 1, dtype=np.uint32) * 3,
+                                         precision=jne.PRECISIONS["mfma_fp16x2"])
+        assert np.array_equal(got, ref)
+
+
 @pytest.mark.parametrize("model,dim,steps,n", [(0, 2, 1000, 20000), (2, 5, 5000, 2000), (4, 12, 10000, 400)])
 def test_workload_shapes_many_walks(engine, O, model, dim, steps, n):
     """The plain-CPU, plain-GPU and headline workload shapes on many walks

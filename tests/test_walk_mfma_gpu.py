@@ -68,7 +68,7 @@ def tol_for(prec, steps):
 
 @pytest.mark.parametrize("prec", PRECS)
 @pytest.mark.parametrize("model", [0, 1, 2, 3, 4])
-@pytest.mark.parametrize("dim", [1, 2, 5, 7, 12])
+@pytest.mark.parametrize("dim", list(range(1, 13)))
 def test_mixed_matches_fp64_path(engine, jne, prec, model, dim):
     # steps: < one segment, exactly one, several plus a tail
     for steps in [33, 64, 200, 1000]:
@@ -102,7 +102,10 @@ def test_mixed_relative_tolerance_headline_batch(engine, jne, prec):
     assert rel.max() <= REL_HEADLINE[prec], f"{prec}: max relative {rel.max():.2e}"
 
 
-@pytest.mark.parametrize("model,dim,steps", [(2, 5, 5000), (1, 7, 3001), (3, 12, 2047), (0, 9, 1100)])
+@pytest.mark.parametrize("model,dim,steps", [(2, 5, 5000), (1, 7, 3001), (3, 12, 2047), (0, 9, 1100)]
+                         # each dim auto sends to the MFMA pass, with a partial last segment
+                         + [(m, 8, 3037) for m in range(5)] + [(m, 10, 333) for m in range(5)]
+                         + [(m, 11, 301) for m in range(5)] + [(m, 9, 517) for m in range(5)])
 def test_mixed_matches_oracle_long(engine, jne, O, model, dim, steps):
     seeds = np.array([5, 6, 7], dtype=np.uint32)
     ref = O.walks_by_seed_c(model, dim, steps, seeds, threads=3)
