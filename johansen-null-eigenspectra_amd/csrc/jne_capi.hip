@@ -217,9 +217,8 @@ int run_walks(jne_ctx* ctx, int model, int dim, uint32_t steps, uint32_t rng_str
     const uint32_t* s = d_seeds ? d_seeds + off : nullptr;
     const double* z = d_z ? d_z + off * (size_t)steps * dim : nullptr;
     JNE_HIP_CHECK(ctx, timed_launch(ctx, JNE_KERNEL_MOMENTS, cn, st, [&] {
-                    if (precision == JNE_PREC_FP64 || precision == JNE_PREC_FP32_KAHAN)
-                      return jne::launch_walk_moments_fp64(dim, precision == JNE_PREC_FP32_KAHAN, s, z, rng_stream,
-                                                           steps, cn, ctx->d_acc, cn, st);
+                    if (precision == JNE_PREC_FP64)
+                      return jne::launch_walk_moments_fp64(dim, s, z, rng_stream, steps, cn, ctx->d_acc, cn, st);
                     return jne::launch_walk_moments_mfma(dim, precision, s, z, rng_stream, steps, cn, ctx->d_acc,
                                                          cn, st);
                   }));

@@ -86,9 +86,9 @@ constexpr int moment_count(int d) { return 2 * d * d + 6 * d; }
 
 constexpr int num_eigenvalues(int model, int d) { return (model == 1 || model == 3) ? d + 1 : d; }
 
-// kahan: the compensated FP32 accumulation variant of the same kernel
-hipError_t launch_walk_moments_fp64(int d, bool kahan, const uint32_t* seeds, const double* zin, uint32_t rng_stream,
-                                    uint32_t T, uint32_t n, double* acc, size_t ld, hipStream_t st);
+hipError_t launch_walk_moments_fp64(int d, const uint32_t* seeds, const double* zin, uint32_t rng_stream, uint32_t T,
+                                    uint32_t n, double* acc, size_t ld, hipStream_t st);
+// the segment-split pass: the MFMA precisions and the compensated FP32 VALU check
 hipError_t launch_walk_moments_mfma(int d, int precision, const uint32_t* seeds, const double* zin,
                                     uint32_t rng_stream, uint32_t T, uint32_t n, double* acc, size_t ld,
                                     hipStream_t st);

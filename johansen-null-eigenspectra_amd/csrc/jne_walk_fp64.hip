@@ -8,8 +8,7 @@
 //   GeneralizedEigen (dggev)    src/johansen_statistics.rs:35-45
 // without materialising B, dB or F.  Kernel 1 streams the walk once and keeps
 // the moments of x_t = [b_t, 1, u_t, u_t^2, e_t] (b_t = B_t before step t,
-// e_t = B_{t+1} - B_t, u_t = (t+1)/T) in FP64 (or, as the compensated check,
-// FP32 products in Kahan-compensated FP32 sums); kernel 2 forms S11 and S01 of
+// e_t = B_{t+1} - B_t, u_t = (t+1)/T) in FP64; kernel 2 forms S11 and S01 of
 // the requested model from those moments (demeaning and detrending are Schur
 // complements on the {1, u} columns) and solves the eigenproblem -- as one
 // kernel, or for the larger dims as two (algebra up to M = W^T W, then Jacobi).
@@ -52,14 +51,8 @@ namespace jne {
 // The walk index is the fastest LDS dimension (odd pitch) so that the lanes
 // of a wave read from different banks.
 // ---------------------------------------------------------------------------
-// Gram accumulator: FP64 fma, or (the compensated FP32 variant) an FP32
-// product added with Kahan's compensated summation.  Both take 8 bytes.
-template <bool KAHAN>
-struct GramAcc;
-
-template <>
-struct GramAcc<false> {
-  using Real = double;
+// FP64 Gram accumulator.
+struct GramAcc {
   double s;
   __device__ __forceinline__ void zero() { s = 0.0; }
   __device__ __forceinline__ void add_prod(double a, double b) { s = fma(a, b, s); }
@@ -67,35 +60,11 @@ struct GramAcc<false> {
   __device__ __forceinline__ double value() const { return s; }
 };
 
-template <>
-struct GramAcc<true> {
-  using Real = float;
-  float s, c;  // sum and the negated low part lost so far
-  __device__ __forceinline__ void zero() { s = c = 0.0f; }
-  __device__ __forceinline__ void add(float x) {
-    const float y = x - c;
-    const float t = s + y;
-    c = (t - s) - y;
-    s = t;
-  }
-  __device__ __forceinline__ void add_prod(float a, float b) { add(a * b); }
-  __device__ __forceinline__ double value() const { return (double)s - (double)c; }
-};
-
-// The compensated variant keeps an FP32 copy of the state beside the FP64 one
-// and needs more registers: one wave per SIMD fewer where the FP64 target is
-// already tight.
-constexpr int kahan_waves_for_dim(int d) {
-  return (d == 8 || d >= 11) ? 1 : (d == 6 || d == 10) ? 2 : waves_for_dim(d);
-}
-
-template <int D, bool FROM_NORMALS, bool KAHAN>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KAHAN ? kahan_waves_for_dim(D) : waves_for_dim(D),
-                                                                   KAHAN ? kahan_waves_for_dim(D) : waves_for_dim(D))))
+template <int D, bool FROM_NORMALS>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(waves_for_dim(D), waves_for_dim(D))))
 void walk_moments_fp64(const uint32_t* __restrict__ seeds, const double* __restrict__ zin, uint32_t rng_stream,
                        uint32_t T, uint32_t n_walks, double* __restrict__ acc, size_t ld) {
-  using Acc = GramAcc<KAHAN>;
-  using Real = typename Acc::Real;
+  using Acc = GramAcc;
   using Sh = WalkShape<D>;
   constexpr int L = Sh::L;
   constexpr int DP = Sh::DP;
@@ -216,26 +185,18 @@ void walk_moments_fp64(const uint32_t* __restrict__ seeds, const double* __restr
         bnk[k] = b[k] + w[k * GP];  // B_{t+1} = B_t + z*sqrt(dt) (rng_matrix.rs:138-140)
         e[k] = bnk[k] - b[k];  // dbm = bm_current - bm_previous (johansen_statistics.rs:80-82)
       }
-      // the Gram operands in the accumulator's precision (FP32 copies of the
-      // FP64 state for the compensated variant)
-      Real bq[DP], eq[R];
-#pragma unroll
-      for (int m = 0; m < DP; ++m) bq[m] = (Real)b[m];
-#pragma unroll
-      for (int k = 0; k < R; ++k) eq[k] = (Real)e[k];
-      const Real uq = (Real)u, u2q = (Real)u2;
 #pragma unroll
       for (int k = 0; k < R; ++k) {
 #pragma unroll
-        for (int m = 0; m < DP; ++m) geb[k][m].add_prod(eq[k], bq[m]);
+        for (int m = 0; m < DP; ++m) geb[k][m].add_prod(e[k], b[m]);
 #pragma unroll
-        for (int dl = 0; dl <= H; ++dl) gbb[k][dl].add_prod(bq[k], bq[(k + dl) % DP]);
-        mom[0][k].add(bq[k]);
-        mom[1][k].add_prod(uq, bq[k]);
-        mom[2][k].add_prod(u2q, bq[k]);
-        mom[3][k].add(eq[k]);
-        mom[4][k].add_prod(uq, eq[k]);
-        mom[5][k].add_prod(u2q, eq[k]);
+        for (int dl = 0; dl <= H; ++dl) gbb[k][dl].add_prod(b[k], b[(k + dl) % DP]);
+        mom[0][k].add(b[k]);
+        mom[1][k].add_prod(u, b[k]);
+        mom[2][k].add_prod(u2, b[k]);
+        mom[3][k].add(e[k]);
+        mom[4][k].add_prod(u, e[k]);
+        mom[5][k].add_prod(u2, e[k]);
       }
 #pragma unroll
       for (int k = 0; k < R; ++k) b[k] = bnk[k];
@@ -590,25 +551,18 @@ __global__ __launch_bounds__(64) void walk_eigen_jacobi(const double* __restrict
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <int D, bool KAHAN>
-static hipError_t launch_moments_dk(const uint32_t* seeds, const double* zin, uint32_t rng_stream, uint32_t T,
-                                    uint32_t n, double* acc, size_t ld, hipStream_t st) {
+template <int D>
+static hipError_t launch_moments_d(const uint32_t* seeds, const double* zin, uint32_t rng_stream, uint32_t T,
+                                   uint32_t n, double* acc, size_t ld, hipStream_t st) {
   constexpr int GPW = WalkShape<D>::GPW;
   const unsigned grid = (unsigned)((n + GPW - 1) / GPW);
   if (zin != nullptr)
-    hipLaunchKernelGGL((walk_moments_fp64<D, true, KAHAN>), dim3(grid), dim3(64), 0, st, seeds, zin, rng_stream, T,
-                       n, acc, ld);
+    hipLaunchKernelGGL((walk_moments_fp64<D, true>), dim3(grid), dim3(64), 0, st, seeds, zin, rng_stream, T, n, acc,
+                       ld);
   else
-    hipLaunchKernelGGL((walk_moments_fp64<D, false, KAHAN>), dim3(grid), dim3(64), 0, st, seeds, zin, rng_stream,
-                       T, n, acc, ld);
+    hipLaunchKernelGGL((walk_moments_fp64<D, false>), dim3(grid), dim3(64), 0, st, seeds, zin, rng_stream, T, n,
+                       acc, ld);
   return hipGetLastError();
-}
-
-template <int D>
-static hipError_t launch_moments_d(bool kahan, const uint32_t* seeds, const double* zin, uint32_t rng_stream,
-                                   uint32_t T, uint32_t n, double* acc, size_t ld, hipStream_t st) {
-  return kahan ? launch_moments_dk<D, true>(seeds, zin, rng_stream, T, n, acc, ld, st)
-               : launch_moments_dk<D, false>(seeds, zin, rng_stream, T, n, acc, ld, st);
 }
 
 template <int MODEL, int D>
@@ -628,12 +582,12 @@ static hipError_t launch_eigen_md(const double* acc, size_t ld, uint32_t T, uint
   return hipGetLastError();
 }
 
-hipError_t launch_walk_moments_fp64(int d, bool kahan, const uint32_t* seeds, const double* zin, uint32_t rng_stream,
-                                    uint32_t T, uint32_t n, double* acc, size_t ld, hipStream_t st) {
+hipError_t launch_walk_moments_fp64(int d, const uint32_t* seeds, const double* zin, uint32_t rng_stream, uint32_t T,
+                                    uint32_t n, double* acc, size_t ld, hipStream_t st) {
   switch (d) {
 #define JNE_CASE(DD) \
   case DD:           \
-    return launch_moments_d<DD>(kahan, seeds, zin, rng_stream, T, n, acc, ld, st);
+    return launch_moments_d<DD>(seeds, zin, rng_stream, T, n, acc, ld, st);
     JNE_CASE(1) JNE_CASE(2) JNE_CASE(3) JNE_CASE(4) JNE_CASE(5) JNE_CASE(6)
     JNE_CASE(7) JNE_CASE(8) JNE_CASE(9) JNE_CASE(10) JNE_CASE(11) JNE_CASE(12)
 #undef JNE_CASE

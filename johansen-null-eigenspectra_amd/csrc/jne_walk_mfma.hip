@@ -219,6 +219,45 @@ struct ChunkMfma<JNE_PREC_MFMA_FP32> {
   }
 };
 
+// The compensated FP32 VALU check (JNE_PREC_FP32_KAHAN) runs the same
+// segment split with the MFMA Gram replaced by FP32 products of the same X
+// rows (z, c, 1, k, k(k-1)/2) added into Kahan-compensated FP32 sums, one
+// per Gram entry the flush reads: for each c column the z, c and constant
+// rows, for each z column the rows k and k(k-1)/2.  Lane l owns entries
+// l, l + 64, ...; each segment's sums go to an LDS tile of doubles
+// (sum - compensation, formed in FP64) that every lane reads in the MFMA
+// output layout, so the flush is shared with the MFMA precisions.
+template <int D>
+struct KahanShape {
+  static constexpr int PER_C = 2 * D + 3;            // rows per c column
+  static constexpr int NE = D * PER_C + 2 * D;       // entries per segment
+  static constexpr int NQ = (NE + 63) / 64;          // entries per lane
+  static constexpr int COLS = kRowC + D;             // tile columns (X rows 0..11+D)
+};
+
+// X rows (i, j) of entry e of the compensated Gram (see KahanShape).
+template <int D>
+__device__ __forceinline__ void kahan_entry(int e, int& i, int& j) {
+  using K = KahanShape<D>;
+  if (e < D * K::PER_C) {
+    const int jc = e / K::PER_C, r = e - jc * K::PER_C;
+    j = kRowC + jc;
+    i = r < D ? r : r < 2 * D ? kRowC + (r - D) : kRowDet + (r - 2 * D);
+  } else {
+    const int ez = e - D * K::PER_C;
+    j = ez >> 1;
+    i = kRowDet + 1 + (ez & 1);
+  }
+}
+
+__device__ __forceinline__ void kahan_add(float& s, float& c, float x) {
+#pragma clang fp contract(off)
+  const float y = x - c;
+  const float t = s + y;
+  c = (t - s) - y;
+  s = t;
+}
+
 // Philox calls of a segment unrolled: by-seed form only (the from-normals
 // form has no Philox chains to overlap), whole iterations only, and not with
 // the three-way bf16 split, whose operands leave no registers for it
@@ -249,12 +288,15 @@ __device__ __forceinline__ uint32_t xcd_walk(uint32_t b, uint32_t n) {
 }
 
 template <int D, int PREC, bool FROM_NORMALS>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(JNE_MFMA_WAVES)))
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(PREC == JNE_PREC_FP32_KAHAN ? 2 : JNE_MFMA_WAVES)))
 void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restrict__ zin, uint32_t rng_stream,
                        uint32_t T, uint32_t n_walks, double* __restrict__ acc, size_t ld) {
   using Sh = MfmaShape<D>;
   constexpr int ZP = Sh::ZP, NG = Sh::NG, SP = Sh::SP, BS = Sh::BS;
-  using Op = ChunkMfma<PREC>;
+  constexpr bool kKahan = PREC == JNE_PREC_FP32_KAHAN;
+  using Op = ChunkMfma<kKahan ? JNE_PREC_MFMA_FP32 : PREC>;
+  using KS = KahanShape<D>;
 
   // Row r of X lives in LDS row r: the segment's normals z (rows 0..D-1), its
   // c rows (12..12+D-1) and the constant rows 1, k, k(k-1)/2 (24..26), written
@@ -268,7 +310,7 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
   double* const gram = reinterpret_cast<double*>(rows_raw);
   __shared__ double eps_part[SP][D];
   // (not with the three-way bf16 split, whose operands leave no registers for it)
-  constexpr bool kCarry = Sh::CARRY && !FROM_NORMALS && PREC != JNE_PREC_MFMA_BF16X3;
+  constexpr bool kCarry = Sh::CARRY && !FROM_NORMALS && PREC != JNE_PREC_MFMA_BF16X3 && PREC != JNE_PREC_FP32_KAHAN;
   __shared__ float4 ovf[kCarry ? Sh::OVF : 1];  // calls drawn ahead for the next segment
   __shared__ double seg_eps[D];
   __shared__ double b0s[D];  // B at the segment start (FP64)
@@ -278,6 +320,8 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
   __shared__ SegVec seg[D];
   __shared__ double mb[3 * D];   // sum over segments of B0' sum_seg u^p (lanes l < 3D)
   __shared__ double gd[2 * D][3];  // Gram rows 24..26 of the c columns, then of the z columns
+  // the compensated check's per-segment Gram entries, [X row][X column]
+  __shared__ double gk[kKahan ? kRowDet + 3 : 1][kKahan ? KS::COLS : 1];
 
   const int l = threadIdx.x;
   const int row = l & 31;  // also the accumulator column
@@ -335,6 +379,19 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
   double bbA[NG], ebA[NG];
 #pragma unroll
   for (int q = 0; q < NG; ++q) bbA[q] = ebA[q] = 0.0;
+  // the compensated check: this lane's Gram entries as LDS row offsets; the
+  // entries no lane computes (rows of unused components) stay zero
+  int kei[kKahan ? KS::NQ : 1], kej[kKahan ? KS::NQ : 1];
+  if constexpr (kKahan) {
+    for (int e = l; e < (kRowDet + 3) * KS::COLS; e += 64) (&gk[0][0])[e] = 0.0;
+#pragma unroll
+    for (int q = 0; q < KS::NQ; ++q) {
+      int i = 0, j = 0;
+      if (l + 64 * q < KS::NE) kahan_entry<D>(l + 64 * q, i, j);
+      kei[q] = i;
+      kej[q] = j;
+    }
+  }
   const int mp = l / D, mi = l % D;  // moment vector entry (power mp, component mi) for l < 3D
   const CallPos<D> pos0(l);          // normals of this lane's first Philox call in a segment
   // scan lane (component si, block sp): steps [sp BS, sp BS + BS) of a segment
@@ -475,27 +532,57 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
     }
     __syncthreads();
 
-    // ---- 4 chunks of 16 steps on the matrix cores
-    floatx16 accm = {}, accc = {};
-#pragma unroll JNE_MFMA_CHUNK_UNROLL
-    for (int ch = 0; ch < kChunks; ++ch) {
-      const int off = 16 * ch;
-      float x[8];
+    // ---- the segment's Gram of X: f[v] is element [8(v/4) + 4h + v%4][l%32]
+    // (the 32x32 MFMA output layout)
+    double f[16];
+    if constexpr (kKahan) {
+      // FP32 products in Kahan-compensated FP32 sums, then to LDS
+      float ks[KS::NQ], kc[KS::NQ];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) x[k] = xrow[off + (Op::kInterleaved ? 2 * k + h : 8 * h + k)];
-      Op::run(x, accm, accc);
+      for (int q = 0; q < KS::NQ; ++q) ks[q] = kc[q] = 0.0f;
+#pragma unroll 2
+      for (int k4 = 0; k4 < kSegSteps / 4; ++k4) {
+#pragma unroll
+        for (int q = 0; q < KS::NQ; ++q) {
+          const float4 a = *reinterpret_cast<const float4*>(&xs[kei[q]][4 * k4]);
+          const float4 b = *reinterpret_cast<const float4*>(&xs[kej[q]][4 * k4]);
+          kahan_add(ks[q], kc[q], a.x * b.x);
+          kahan_add(ks[q], kc[q], a.y * b.y);
+          kahan_add(ks[q], kc[q], a.z * b.z);
+          kahan_add(ks[q], kc[q], a.w * b.w);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < KS::NQ; ++q)
+        if (l + 64 * q < KS::NE) gk[kei[q]][kej[q]] = (double)ks[q] - (double)kc[q];
+      __syncthreads();
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int i = 8 * (v / 4) + 4 * h + v % 4;
+        f[v] = (i < kRowDet + 3 && row < KS::COLS) ? gk[i][row] : 0.0;
+      }
+    } else {
+      // 4 chunks of 16 steps on the matrix cores
+      floatx16 accm = {}, accc = {};
+#pragma unroll JNE_MFMA_CHUNK_UNROLL
+      for (int ch = 0; ch < kChunks; ++ch) {
+        const int off = 16 * ch;
+        float x[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) x[k] = xrow[off + (Op::kInterleaved ? 2 * k + h : 8 * h + k)];
+        Op::run(x, accm, accc);
+      }
+#pragma unroll
+      for (int v = 0; v < 16; ++v) f[v] = (double)(accm[v] + accc[v]);
     }
 
     // ---- segment end
-    float f[16];
-#pragma unroll
-    for (int v = 0; v < 16; ++v) f[v] = accm[v] + accc[v];
     // f[12] on the lower half is row 24 (the ones row): sum_seg x_col
     const uint32_t te = tail ? T : ts + kSegSteps;
     const double nseg = (double)(te - ts);
     if (h == 0 && is_c) {
       SegVec& v = seg[row - kRowC];
-      const double sig = (double)f[12];
+      const double sig = f[12];
       v.P = nseg * v.b + s * sig;
       v.Q = s * sig;
     }
@@ -505,10 +592,10 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
       // sum x, sum k x, sum k(k-1)/2 x: stored as sum x, sum u x, sum u^2 x with
       // u = (a + k)/T, a = ts + 1; on the z columns sum x is the FP64 eps.
       const double a = (double)ts + 1.0;
-      const double P0 = is_z ? seg_eps[row] : (double)f[12];
-      const double P1 = (double)f[13], P2 = (double)f[14];
+      const double P0 = is_z ? seg_eps[row] : f[12];
+      const double P1 = f[13], P2 = f[14];
 #pragma unroll
-      for (int v = 0; v < 12; ++v) g[v] += (double)f[v];
+      for (int v = 0; v < 12; ++v) g[v] += f[v];
       if (det_lane) {
         gdl[0] += P0;
         gdl[1] += (a * P0 + P1) * dt;
@@ -630,6 +717,7 @@ hipError_t launch_walk_moments_mfma(int d, int precision, const uint32_t* seeds,
     case JNE_PREC_MFMA_BF16X2: return launch_mfma_p<JNE_PREC_MFMA_BF16X2>(d, seeds, zin, rng_stream, T, n, acc, ld, st);
     case JNE_PREC_MFMA_BF16X3: return launch_mfma_p<JNE_PREC_MFMA_BF16X3>(d, seeds, zin, rng_stream, T, n, acc, ld, st);
     case JNE_PREC_MFMA_FP32: return launch_mfma_p<JNE_PREC_MFMA_FP32>(d, seeds, zin, rng_stream, T, n, acc, ld, st);
+    case JNE_PREC_FP32_KAHAN: return launch_mfma_p<JNE_PREC_FP32_KAHAN>(d, seeds, zin, rng_stream, T, n, acc, ld, st);
     default: return hipErrorInvalidValue;
   }
 }

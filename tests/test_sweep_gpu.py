@@ -35,7 +35,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CLI = os.path.join(ROOT, "johansen-null-eigenspectra_amd", "bin", "johansen-null-eigenspectra")
 ORACLE_SEEDS = np.array([1, 2, 2_654_435_761, 0xFFFFFFFF], dtype=np.uint32)
 N_MIXED = 256
-TOL_MIXED = {"mfma_fp16x2": 1e-5, "mfma_bf16x3": 2e-5, "mfma_fp32": 1e-5}
+TOL_MIXED = {"mfma_fp16x2": 1e-5, "mfma_bf16x3": 2e-5, "mfma_fp32": 1e-5, "fp32_kahan": 1e-5}
 PAIRS = [(m, d) for m in range(5) for d in range(1, 13)]
 
 
