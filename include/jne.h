@@ -103,6 +103,26 @@ int jne_eigenvalues_by_seed_device(jne_ctx *ctx, int model, int dim, uint32_t st
                                    const uint32_t *d_seeds, size_t n, int precision, double *d_out,
                                    void *hip_stream);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The same as jne_eigenvalues_by_seed_device, with the
+ * same kernels and bit-identical results, but its per-walk scratch (the
+ * moments, 8 (2 dim^2 + 6 dim) bytes per walk, and the M matrices of the eigen
+ * solve, 8 (dim (dim + 1) / 2 + 1) bytes) lives in a fixed device array of
+ * JNE_NOALLOC_SCRATCH_BYTES inside the library, one per GPU, held until the
+ * process ends; the call allocates nothing.  n is at most
+ * JNE_NOALLOC_SCRATCH_BYTES / (8 (2.5 dim^2 + 6.5 dim + 1)) walks (19,108 at
+ * dim 12, 838,860 at dim 1); larger n is JNE_ERR_INVALID_ARG with a message.
+ * Every call on one GPU shares that GPU's array (whatever the context), and
+ * the array keeps the last call's contents: calls that overlap on one GPU
+ * (other streams, other host threads, other contexts) race on it, so the
+ * caller orders them.  Nothing in the array needs to start at zero.
+ * Profiling (jne_profile_enable) does not record these launches. */
+#define JNE_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int jne_eigenvalues_by_seed_device_noalloc(jne_ctx *ctx, int model, int dim, uint32_t steps, uint32_t rng_stream,
+                                           const uint32_t *d_seeds, size_t n, int precision, double *d_out,
+                                           void *hip_stream);
+
 /* Parity harness: eigenvalues from caller-supplied N(0,1) increments.
  * z is a host array of n * steps * dim doubles, z[(w*steps + t)*dim + i]
  * (component fastest, like the reference's column-major dim x T draw,

@@ -71,6 +71,15 @@ PRECISIONS = {
 
 MAX_DIM = 12
 
+# JNE_NOALLOC_SCRATCH_BYTES: the fixed per-GPU scratch of the no-alloc entry point
+NOALLOC_SCRATCH_BYTES = 64 << 20
+
+
+def noalloc_max_walks(dim: int) -> int:
+    """Walks one no-alloc call takes at `dim`: the scratch over 8 (2.5 dim^2 + 6.5 dim + 1) bytes."""
+    d = int(dim)
+    return (NOALLOC_SCRATCH_BYTES // 8) // (2 * d * d + 6 * d + d * (d + 1) // 2 + 1)
+
 KERNEL_MOMENTS = 0
 KERNEL_EIGEN = 1
 KERNEL_NORMALS = 2
@@ -120,6 +129,8 @@ def load_library() -> ctypes.CDLL:
                                                      ctypes.c_uint32, ctypes.c_void_p, ctypes.c_size_t,
                                                      ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
         L.jne_eigenvalues_by_seed_device.restype = ctypes.c_int
+        L.jne_eigenvalues_by_seed_device_noalloc.argtypes = L.jne_eigenvalues_by_seed_device.argtypes
+        L.jne_eigenvalues_by_seed_device_noalloc.restype = ctypes.c_int
         L.jne_eigenvalues_from_normals.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
                                                    ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p]
         L.jne_eigenvalues_from_normals.restype = ctypes.c_int
@@ -310,6 +321,15 @@ class Engine:
         """Device pointers (ints); enqueues on hip_stream (None: the context's stream; 0: HIP's null
         stream, e.g. torch's default stream) without synchronising."""
         self._check(self._lib.jne_eigenvalues_by_seed_device(
+            self._ctx, int(model), int(dim), int(steps), int(rng_stream), ctypes.c_void_p(d_seeds), int(n),
+            int(precision), ctypes.c_void_p(d_out), _stream_arg(hip_stream)))
+
+    def eigenvalues_by_seed_device_noalloc(self, model: int, dim: int, steps: int, d_seeds: int, n: int, d_out: int,
+                                           rng_stream: int = 0, precision: int = PREC_FP64, hip_stream=None):
+        """eigenvalues_by_seed_device without any allocation: the scratch is the library's fixed
+        per-GPU array (NOALLOC_SCRATCH_BYTES), shared by every such call on the GPU (overlapping calls
+        race); n is at most noalloc_max_walks(dim)."""
+        self._check(self._lib.jne_eigenvalues_by_seed_device_noalloc(
             self._ctx, int(model), int(dim), int(steps), int(rng_stream), ctypes.c_void_p(d_seeds), int(n),
             int(precision), ctypes.c_void_p(d_out), _stream_arg(hip_stream)))
 

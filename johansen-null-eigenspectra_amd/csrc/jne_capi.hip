@@ -61,9 +61,25 @@ hipError_t launch_walk_normals(int d, const uint32_t* seeds, uint32_t rng_stream
 
 }  // namespace jne
 
+// Device scratch of jne_eigenvalues_by_seed_device_noalloc: the per-walk
+// moments and the M matrices of the two-kernel eigen solve of one call, in a
+// fixed array that each GPU the library runs on holds until the process ends
+// (JNE_NOALLOC_SCRATCH_BYTES).  In host code `scratch` names a host-side
+// stand-in, not the device address: the address of each GPU's copy is taken
+// once per device with hipGetSymbolAddress (jne_create) and only that
+// pointer is passed to kernels.
+static __device__ __align__(256) double scratch[JNE_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};  // per device: the address of its `scratch`
+}  // namespace
+
 struct jne_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  double* noalloc_scratch = nullptr;  // this device's `scratch`
   std::mutex mu;
   std::string err;
   // device scratch: per-walk moments of one chunk
@@ -199,10 +215,37 @@ bool auto_uses_mfma(int dim, uint32_t steps) {
   return false;
 }
 
-int run_walks(jne_ctx* ctx, int model, int dim, uint32_t steps, uint32_t rng_stream, const uint32_t* d_seeds,
-              const double* d_z, size_t n, int precision, double* d_out, hipStream_t st) {
+// Enqueues the walk kernels for n walks on st, in chunks of at most `chunk`
+// walks over the scratch `acc` (chunk * moment_count(dim) doubles) and `mbuf`
+// (chunk * eigen_m_count(dim) doubles).  timed: record the launches when
+// profiling is enabled.
+int launch_walks(jne_ctx* ctx, int model, int dim, uint32_t steps, uint32_t rng_stream, const uint32_t* d_seeds,
+                 const double* d_z, size_t n, int precision, double* d_out, hipStream_t st, double* acc,
+                 double* mbuf, size_t chunk, bool timed) {
   if (precision == JNE_PREC_AUTO) precision = auto_uses_mfma(dim, steps) ? JNE_PREC_MFMA_FP16X2 : JNE_PREC_FP64;
   const int nf = jne::num_eigenvalues(model, dim);
+  for (size_t off = 0; off < n; off += chunk) {
+    const uint32_t cn = (uint32_t)std::min(chunk, n - off);
+    const uint32_t* s = d_seeds ? d_seeds + off : nullptr;
+    const double* z = d_z ? d_z + off * (size_t)steps * dim : nullptr;
+    auto moments = [&] {
+      if (precision == JNE_PREC_FP64) return jne::launch_walk_moments_fp64(dim, s, z, rng_stream, steps, cn, acc, cn, st);
+      return jne::launch_walk_moments_mfma(dim, precision, s, z, rng_stream, steps, cn, acc, cn, st);
+    };
+    auto eigen = [&] { return jne::launch_walk_eigen_fp64(model, dim, acc, cn, steps, cn, d_out + off * nf, mbuf, st); };
+    if (timed) {
+      JNE_HIP_CHECK(ctx, timed_launch(ctx, JNE_KERNEL_MOMENTS, cn, st, moments));
+      JNE_HIP_CHECK(ctx, timed_launch(ctx, JNE_KERNEL_EIGEN, cn, st, eigen));
+    } else {
+      JNE_HIP_CHECK(ctx, moments());
+      JNE_HIP_CHECK(ctx, eigen());
+    }
+  }
+  return JNE_OK;
+}
+
+int run_walks(jne_ctx* ctx, int model, int dim, uint32_t steps, uint32_t rng_stream, const uint32_t* d_seeds,
+              const double* d_z, size_t n, int precision, double* d_out, hipStream_t st) {
   const size_t mc = (size_t)jne::moment_count(dim);
   const size_t chunk = std::min(n, kChunkWalks);
   int rc = ensure(ctx, (void**)&ctx->d_acc, &ctx->acc_bytes, chunk * mc * sizeof(double));
@@ -212,21 +255,9 @@ int run_walks(jne_ctx* ctx, int model, int dim, uint32_t steps, uint32_t rng_str
   if (!ctx->scratch_free) JNE_HIP_CHECK(ctx, hipEventCreateWithFlags(&ctx->scratch_free, hipEventDisableTiming));
   if (ctx->scratch_stream && ctx->scratch_stream != st)
     JNE_HIP_CHECK(ctx, hipStreamWaitEvent(st, ctx->scratch_free, 0));
-  for (size_t off = 0; off < n; off += chunk) {
-    const uint32_t cn = (uint32_t)std::min(chunk, n - off);
-    const uint32_t* s = d_seeds ? d_seeds + off : nullptr;
-    const double* z = d_z ? d_z + off * (size_t)steps * dim : nullptr;
-    JNE_HIP_CHECK(ctx, timed_launch(ctx, JNE_KERNEL_MOMENTS, cn, st, [&] {
-                    if (precision == JNE_PREC_FP64)
-                      return jne::launch_walk_moments_fp64(dim, s, z, rng_stream, steps, cn, ctx->d_acc, cn, st);
-                    return jne::launch_walk_moments_mfma(dim, precision, s, z, rng_stream, steps, cn, ctx->d_acc,
-                                                         cn, st);
-                  }));
-    JNE_HIP_CHECK(ctx, timed_launch(ctx, JNE_KERNEL_EIGEN, cn, st, [&] {
-                    return jne::launch_walk_eigen_fp64(model, dim, ctx->d_acc, cn, steps, cn, d_out + off * nf,
-                                                       ctx->d_m, st);
-                  }));
-  }
+  if ((rc = launch_walks(ctx, model, dim, steps, rng_stream, d_seeds, d_z, n, precision, d_out, st, ctx->d_acc,
+                         ctx->d_m, chunk, true)))
+    return rc;
   JNE_HIP_CHECK(ctx, hipEventRecord(ctx->scratch_free, st));
   ctx->scratch_stream = st;
   return JNE_OK;
@@ -268,12 +299,33 @@ int jne_create(int device, jne_ctx** out) {
     g_create_err = "jne_create: device index out of range";
     return JNE_ERR_INVALID_ARG;
   }
+  if (device >= kMaxDevices) {
+    g_create_err = "jne_create: device index above 63";
+    return JNE_ERR_INVALID_ARG;
+  }
   jne_ctx* ctx = new jne_ctx();
   ctx->device = device;
   if ((e = hipSetDevice(device)) != hipSuccess || (e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess) {
     g_create_err = std::string("jne_create: ") + hipGetErrorString(e);
     delete ctx;
     return JNE_ERR_HIP;
+  }
+  {
+    // the address of this device's copy of the no-alloc scratch, once per
+    // device (with the device current)
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+      void* p = nullptr;
+      e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !p) {
+        g_create_err = std::string("jne_create: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return JNE_ERR_HIP;
+      }
+      g_scratch_ptr[device] = static_cast<double*>(p);
+    }
+    ctx->noalloc_scratch = g_scratch_ptr[device];
   }
   *out = ctx;
   return JNE_OK;
@@ -334,6 +386,31 @@ int jne_eigenvalues_by_seed_device(jne_ctx* ctx, int model, int dim, uint32_t st
   JNE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   hipStream_t st = stream_of(ctx, hip_stream);
   return run_walks(ctx, model, dim, steps, rng_stream, d_seeds, nullptr, n, precision, d_out, st);
+}
+
+int jne_eigenvalues_by_seed_device_noalloc(jne_ctx* ctx, int model, int dim, uint32_t steps, uint32_t rng_stream,
+                                           const uint32_t* d_seeds, size_t n, int precision, double* d_out,
+                                           void* hip_stream) {
+  if (!ctx) return JNE_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->err.clear();
+  int rc = check_args(ctx, model, dim, steps, precision);
+  if (rc) return rc;
+  if (n == 0) return JNE_OK;
+  if (!d_seeds || !d_out) return set_err(ctx, JNE_ERR_INVALID_ARG, "d_seeds/d_out must not be NULL");
+  const size_t mc = (size_t)jne::moment_count(dim);
+  const size_t per_walk = mc + (size_t)jne::eigen_m_count(dim);
+  const size_t cap = (JNE_NOALLOC_SCRATCH_BYTES / sizeof(double)) / per_walk;
+  if (n > cap)
+    return set_err(ctx, JNE_ERR_INVALID_ARG,
+                   "jne_eigenvalues_by_seed_device_noalloc: %zu walks exceed the scratch of %zu walks at dim %d "
+                   "(%zu bytes per walk in %d MiB)",
+                   n, cap, dim, per_walk * sizeof(double), (int)(JNE_NOALLOC_SCRATCH_BYTES >> 20));
+  if (!ctx->noalloc_scratch) return set_err(ctx, JNE_ERR_HIP, "the no-alloc scratch has no device address");
+  JNE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  double* const acc = ctx->noalloc_scratch;
+  return launch_walks(ctx, model, dim, steps, rng_stream, d_seeds, nullptr, n, precision, d_out,
+                      stream_of(ctx, hip_stream), acc, acc + mc * n, n, false);
 }
 
 int jne_eigenvalues_from_normals(jne_ctx* ctx, int model, int dim, uint32_t steps, const double* z, size_t n,
