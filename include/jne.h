@@ -52,7 +52,12 @@ extern "C" {
  * there and FP64 elsewhere (FP64 results meet the mixed-path tolerance
  * trivially). */
 
-#define JNE_MAX_DIM 12
+/* Dims 1..JNE_MAX_DIM run on the device.  The reduced-precision moment
+ * passes (the MFMA modes and JNE_PREC_FP32_KAHAN) cover dims
+ * 1..JNE_MAX_MIXED_DIM; above that every call runs the FP64 path
+ * (JNE_PREC_AUTO included) and a reduced precision is JNE_ERR_UNSUPPORTED. */
+#define JNE_MAX_DIM 32
+#define JNE_MAX_MIXED_DIM 12
 
 typedef struct jne_ctx jne_ctx;
 
@@ -107,12 +112,14 @@ int jne_eigenvalues_by_seed_device(jne_ctx *ctx, int model, int dim, uint32_t st
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc).  The same as jne_eigenvalues_by_seed_device, with the
  * same kernels and bit-identical results, but its per-walk scratch (the
- * moments, 8 (2 dim^2 + 6 dim) bytes per walk, and the M matrices of the eigen
- * solve, 8 (dim (dim + 1) / 2 + 1) bytes) lives in a fixed device array of
- * JNE_NOALLOC_SCRATCH_BYTES inside the library, one per GPU, held until the
- * process ends; the call allocates nothing.  n is at most
- * JNE_NOALLOC_SCRATCH_BYTES / (8 (2.5 dim^2 + 6.5 dim + 1)) walks (19,108 at
- * dim 12, 838,860 at dim 1); larger n is JNE_ERR_INVALID_ARG with a message.
+ * moments, 8 (2 dim^2 + 6 dim) bytes per walk, and for dims up to
+ * JNE_MAX_MIXED_DIM the M matrices of the eigen solve, 8 (dim (dim + 1) / 2 + 1)
+ * bytes) lives in a fixed device array of JNE_NOALLOC_SCRATCH_BYTES inside the
+ * library, one per GPU, held until the process ends; the call allocates
+ * nothing.  n is at most JNE_NOALLOC_SCRATCH_BYTES / (8 (2.5 dim^2 + 6.5 dim + 1))
+ * walks up to dim 12 (19,108 at dim 12, 838,860 at dim 1) and
+ * JNE_NOALLOC_SCRATCH_BYTES / (8 (2 dim^2 + 6 dim)) above (3,744 at dim 32);
+ * larger n is JNE_ERR_INVALID_ARG with a message.
  * Every call on one GPU shares that GPU's array (whatever the context), and
  * the array keeps the last call's contents: calls that overlap on one GPU
  * (other streams, other host threads, other contexts) race on it, so the

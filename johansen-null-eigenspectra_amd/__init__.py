@@ -69,16 +69,19 @@ PRECISIONS = {
     "auto": PREC_AUTO,
 }
 
-MAX_DIM = 12
+MAX_DIM = 32        # JNE_MAX_DIM: dims the device path runs
+MAX_MIXED_DIM = 12  # JNE_MAX_MIXED_DIM: dims of the reduced precisions
 
 # JNE_NOALLOC_SCRATCH_BYTES: the fixed per-GPU scratch of the no-alloc entry point
 NOALLOC_SCRATCH_BYTES = 64 << 20
 
 
 def noalloc_max_walks(dim: int) -> int:
-    """Walks one no-alloc call takes at `dim`: the scratch over 8 (2.5 dim^2 + 6.5 dim + 1) bytes."""
+    """Walks one no-alloc call takes at `dim`: the scratch over 8 (2.5 dim^2 + 6.5 dim + 1) bytes per
+    walk up to dim 12, 8 (2 dim^2 + 6 dim) above (whose eigen solve needs no scratch)."""
     d = int(dim)
-    return (NOALLOC_SCRATCH_BYTES // 8) // (2 * d * d + 6 * d + d * (d + 1) // 2 + 1)
+    m = d * (d + 1) // 2 + 1 if d <= MAX_MIXED_DIM else 0
+    return (NOALLOC_SCRATCH_BYTES // 8) // (2 * d * d + 6 * d + m)
 
 KERNEL_MOMENTS = 0
 KERNEL_EIGEN = 1

@@ -155,12 +155,16 @@ int check_args(jne_ctx* ctx, int model, int dim, uint32_t steps, int precision) 
     return set_err(ctx, JNE_ERR_INVALID_ARG, "invalid steps %u (expected 2..2^30)", steps);
   switch (precision) {
     case JNE_PREC_FP64:
+    case JNE_PREC_AUTO:
+      return JNE_OK;
     case JNE_PREC_MFMA_FP16X2:
     case JNE_PREC_MFMA_BF16X2:
     case JNE_PREC_MFMA_BF16X3:
     case JNE_PREC_MFMA_FP32:
-    case JNE_PREC_AUTO:
     case JNE_PREC_FP32_KAHAN:
+      if (dim > JNE_MAX_MIXED_DIM)
+        return set_err(ctx, JNE_ERR_UNSUPPORTED, "precision %d covers dims 1..%d; dim %d runs the FP64 path",
+                       precision, JNE_MAX_MIXED_DIM, dim);
       return JNE_OK;
     default:
       return set_err(ctx, JNE_ERR_INVALID_ARG, "invalid precision %d", precision);
@@ -209,6 +213,7 @@ hipError_t timed_launch(jne_ctx* ctx, int kernel, uint64_t walks, hipStream_t st
 // (walks/s, model 4, T = 100..10^4): dims 10-12 from 300 steps, dim 9 from 500
 // and dim 8 from 3,000; short walks carry the MFMA pass's per-walk setup.
 bool auto_uses_mfma(int dim, uint32_t steps) {
+  if (dim > JNE_MAX_MIXED_DIM) return false;
   if (dim >= 10) return steps >= 300;
   if (dim == 9) return steps >= 500;
   if (dim == 8) return steps >= 3000;
@@ -229,10 +234,14 @@ int launch_walks(jne_ctx* ctx, int model, int dim, uint32_t steps, uint32_t rng_
     const uint32_t* s = d_seeds ? d_seeds + off : nullptr;
     const double* z = d_z ? d_z + off * (size_t)steps * dim : nullptr;
     auto moments = [&] {
+      if (dim > jne::kMaxDim) return jne::launch_walk_moments_wide(dim, s, z, rng_stream, steps, cn, acc, cn, st);
       if (precision == JNE_PREC_FP64) return jne::launch_walk_moments_fp64(dim, s, z, rng_stream, steps, cn, acc, cn, st);
       return jne::launch_walk_moments_mfma(dim, precision, s, z, rng_stream, steps, cn, acc, cn, st);
     };
-    auto eigen = [&] { return jne::launch_walk_eigen_fp64(model, dim, acc, cn, steps, cn, d_out + off * nf, mbuf, st); };
+    auto eigen = [&] {
+      if (dim > jne::kMaxDim) return jne::launch_walk_eigen_wide(model, dim, acc, cn, steps, cn, d_out + off * nf, st);
+      return jne::launch_walk_eigen_fp64(model, dim, acc, cn, steps, cn, d_out + off * nf, mbuf, st);
+    };
     if (timed) {
       JNE_HIP_CHECK(ctx, timed_launch(ctx, JNE_KERNEL_MOMENTS, cn, st, moments));
       JNE_HIP_CHECK(ctx, timed_launch(ctx, JNE_KERNEL_EIGEN, cn, st, eigen));
@@ -247,7 +256,8 @@ int launch_walks(jne_ctx* ctx, int model, int dim, uint32_t steps, uint32_t rng_
 int run_walks(jne_ctx* ctx, int model, int dim, uint32_t steps, uint32_t rng_stream, const uint32_t* d_seeds,
               const double* d_z, size_t n, int precision, double* d_out, hipStream_t st) {
   const size_t mc = (size_t)jne::moment_count(dim);
-  const size_t chunk = std::min(n, kChunkWalks);
+  // at most 2^18 walks per launch, and at the wide dims about 1 GiB of moments
+  const size_t chunk = std::min(n, std::min(kChunkWalks, std::max<size_t>(65536, (size_t(1) << 27) / mc)));
   int rc = ensure(ctx, (void**)&ctx->d_acc, &ctx->acc_bytes, chunk * mc * sizeof(double));
   if (rc) return rc;
   rc = ensure(ctx, (void**)&ctx->d_m, &ctx->m_bytes, chunk * (size_t)jne::eigen_m_count(dim) * sizeof(double));

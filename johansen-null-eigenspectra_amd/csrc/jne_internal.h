@@ -7,7 +7,8 @@
 
 namespace jne {
 
-constexpr int kMaxDim = 12;
+constexpr int kMaxDim = 12;      // the register-resident kernels (and the MFMA pass)
+constexpr int kMaxWideDim = 32;  // the runtime-dim FP64 path (jne_walk_wide.hip), dims 13..32
 
 // Per-dim shape of the FP64 moment kernel: lanes per walk (L) and target
 // waves per SIMD (the register cap).  Measured on MI355X (model 4, T = 10^4).
@@ -94,9 +95,15 @@ hipError_t launch_walk_moments_mfma(int d, int precision, const uint32_t* seeds,
                                     hipStream_t st);
 // mbuf: (eigen_m_count(d)) * ld doubles of scratch for the two-kernel form
 // (dims whose fused solve would hold fewer than 2 waves per SIMD)
-constexpr int eigen_m_count(int d) { return d * (d + 1) / 2 + 1; }
+// (none for the runtime-dim path, whose solve keeps M in LDS)
+constexpr int eigen_m_count(int d) { return d <= kMaxDim ? d * (d + 1) / 2 + 1 : 0; }
 hipError_t launch_walk_eigen_fp64(int model, int d, const double* acc, size_t ld, uint32_t T, uint32_t n,
                                   double* out, double* mbuf, hipStream_t st);
+// the runtime-dim FP64 path, dims kMaxDim+1 .. kMaxWideDim
+hipError_t launch_walk_moments_wide(int d, const uint32_t* seeds, const double* zin, uint32_t rng_stream, uint32_t T,
+                                    uint32_t n, double* acc, size_t ld, hipStream_t st);
+hipError_t launch_walk_eigen_wide(int model, int d, const double* acc, size_t ld, uint32_t T, uint32_t n, double* out,
+                                  hipStream_t st);
 hipError_t launch_walk_normals(int d, const uint32_t* seeds, uint32_t rng_stream, uint32_t T, uint32_t n,
                                float* out, hipStream_t st);
 

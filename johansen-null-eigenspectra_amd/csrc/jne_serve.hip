@@ -431,6 +431,10 @@ int jne_server_submit(jne_server* s, int model, int dim, uint32_t steps, uint64_
     jne_host::set_error("jne_server_submit: unsupported precision");
     return JNE_ERR_INVALID_ARG;
   }
+  if (dim > JNE_MAX_MIXED_DIM && precision != JNE_PREC_FP64 && precision != JNE_PREC_AUTO) {
+    jne_host::set_error("jne_server_submit: the reduced precisions cover dims 1..12; above, use fp64 or auto");
+    return JNE_ERR_UNSUPPORTED;
+  }
   auto r = std::make_shared<Request>();
   r->model = model;
   r->dim = dim;

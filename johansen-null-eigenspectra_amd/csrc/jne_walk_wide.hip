@@ -1,0 +1,414 @@
+// jne_walk_wide.hip -- FP64 walk path for dims 13..32 (runtime dim).
+//
+// The reference's per-walk operator takes any dim
+// (calculate_eigenvalues, src/johansen_statistics.rs:59-85; the CLI checks
+// only dim > 0, src/cli.rs:201-225).  The register-resident kernels of
+// jne_walk_fp64.hip cover dims 1..12; above that a walk's Gram and its
+// eigenproblem no longer fit one lane's registers, so here one wave owns one
+// walk and keeps both in registers spread over its lanes and in LDS:
+//
+//   walk_moments_wide  the same one-pass moments as walk_moments_fp64 (the
+//                      canonical entry-major layout of jne_internal.h): per
+//                      round of S steps the wave draws the S*dim normals
+//                      (Philox calls dealt over the lanes, the stream of
+//                      jne_rng.h), lanes 0..dim-1 run the FP64 recurrence
+//                      B_{t+1} = B_t + z sqrt(dt) of their component, and every
+//                      lane adds its 8 x 4 tile of sum [b; e] b^T and one row of
+//                      the moment vectors, FP64 fma;
+//   walk_eigen_wide    model algebra (S11 and S01 as Schur complements on the
+//                      moments), Cholesky of S11, W = L^-1 S10, M = W^T W and
+//                      cyclic Jacobi on M, with lanes working on matrix
+//                      entries, rows or columns in parallel.
+//
+// The arithmetic is that of walk_moments_fp64 / walk_eigen_fp64 (FP64
+// throughout, threshold Jacobi), so the results meet the same 1e-9 parity
+// with the reference's dggev solve.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "jne_internal.h"
+#include "jne_rng.h"
+
+namespace jne {
+
+namespace {
+
+constexpr int kPadDim = kMaxWideDim;            // components padded to 32
+constexpr int kRoundMax = 16;                   // most steps in one round (dim 13..16)
+constexpr int kYRow = 2 * kPadDim;              // LDS row of one step: [b (32) | e (32)]
+constexpr int kMaxNx = kMaxWideDim + 1;         // nF of models 1 and 3 at dim 32
+
+// Steps per round: a multiple of 4 (whole Philox calls for any dim) with at
+// most 64 calls, so each lane draws at most one call per round.
+__host__ __device__ constexpr int wide_round_steps(int d) { return ((256 / d) & ~3) < 4 ? 4 : ((256 / d) & ~3); }
+
+}  // namespace
+
+// One wave per walk (blockIdx.x).  Lane l owns rows 8(l/8)..8(l/8)+7 of the
+// stacked rows y = [b (32, zero-padded); e (32)] against columns
+// 4(l%8)..4(l%8)+3 of b: sum y b^T covers sum b b^T and sum e b^T; and the
+// moments sum y, sum u y, sum u^2 y of row l.
+template <bool FROM_NORMALS>
+__global__ __launch_bounds__(64) void walk_moments_wide(const uint32_t* __restrict__ seeds,
+                                                        const double* __restrict__ zin, uint32_t rng_stream,
+                                                        uint32_t T, uint32_t n_walks, int D,
+                                                        double* __restrict__ acc, size_t ld) {
+  __shared__ __align__(16) double ys[kRoundMax][kYRow];  // per step of the round: b, then e
+  __shared__ double wz[kRoundMax * kPadDim];              // the round's increments z sqrt(dt), t-major
+  const int l = threadIdx.x;
+  const size_t walk = blockIdx.x;
+  if (walk >= n_walks) return;  // whole wave exits together
+  const int S = wide_round_steps(D);
+  const int calls = S * D / 4;
+  uint32_t seed = 0;
+  if constexpr (!FROM_NORMALS) seed = seeds[walk];
+  const double* zw = FROM_NORMALS ? zin + walk * (size_t)T * D : nullptr;
+
+  // padding rows and columns stay zero
+  for (int e = l; e < kRoundMax * kYRow; e += 64) (&ys[0][0])[e] = 0.0;
+  __syncthreads();
+
+  const int rb = l >> 3, cb = l & 7;
+  double g[8][4];
+#pragma unroll
+  for (int r = 0; r < 8; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) g[r][c] = 0.0;
+  double m0 = 0.0, m1 = 0.0, m2 = 0.0;  // sum y_l, sum u y_l, sum u^2 y_l
+
+  // src/johansen_statistics.rs:70 and src/rng_matrix.rs:137: sqrt(1/steps)
+  const double dt = 1.0 / (double)T;
+  const double sqrt_dt = sqrt(dt);
+  double b = 0.0;  // B_t of component l (lanes l < D)
+
+  for (uint32_t t0 = 0; t0 < T; t0 += (uint32_t)S) {
+    const int ns = (T - t0) < (uint32_t)S ? (int)(T - t0) : S;
+    // this round's increments: call l draws normals 4l..4l+3 of the round
+    if (l < calls) {
+      float v[4];
+      if constexpr (!FROM_NORMALS) normals4((uint64_t)t0 * D / 4 + l, seed, rng_stream, v);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int q = 4 * l + j;  // = s*D + i
+        double w;
+        if constexpr (FROM_NORMALS) {
+          const uint32_t t = t0 + (uint32_t)(q / D);
+          w = t < T ? zw[(size_t)t0 * D + q] * sqrt_dt : 0.0;
+        } else {
+          w = (double)v[j] * sqrt_dt;
+        }
+        wz[q] = w;
+      }
+    }
+    __syncthreads();
+    // the recurrence of component l, with the reference's rounding:
+    // B_{t+1} = B_t + z sqrt(dt) (rng_matrix.rs:138-140), e = B_{t+1} - B_t
+    // (johansen_statistics.rs:80-82)
+    if (l < D) {
+      for (int s = 0; s < ns; ++s) {
+        const double bn = b + wz[s * D + l];
+        ys[s][l] = b;
+        ys[s][kPadDim + l] = bn - b;
+        b = bn;
+      }
+    }
+    __syncthreads();
+    for (int s = 0; s < ns; ++s) {
+      const double2* yr = reinterpret_cast<const double2*>(&ys[s][8 * rb]);
+      const double2* bc = reinterpret_cast<const double2*>(&ys[s][4 * cb]);
+      double y[8], bb[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const double2 v = yr[k];
+        y[2 * k] = v.x;
+        y[2 * k + 1] = v.y;
+      }
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const double2 v = bc[k];
+        bb[2 * k] = v.x;
+        bb[2 * k + 1] = v.y;
+      }
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) g[r][c] = fma(y[r], bb[c], g[r][c]);
+      const double u = (double)(t0 + (uint32_t)s + 1u) * dt;
+      const double yl = ys[s][l];
+      m0 += yl;
+      m1 = fma(u, yl, m1);
+      m2 = fma(u * u, yl, m2);
+    }
+    __syncthreads();
+  }
+
+  // canonical layout, entry-major: [0, D^2) sum b b^T, [D^2, 2D^2) sum e b^T
+  // (row e, column b), then sum b, sum u b, sum u^2 b, sum e, sum u e, sum u^2 e
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int row = 8 * rb + r;
+    const bool is_e = row >= kPadDim;
+    const int i = is_e ? row - kPadDim : row;
+    if (i >= D) continue;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int j = 4 * cb + c;
+      if (j < D) acc[(size_t)((is_e ? D * D : 0) + i * D + j) * ld + walk] = g[r][c];
+    }
+  }
+  {
+    const bool is_e = l >= kPadDim;
+    const int i = is_e ? l - kPadDim : l;
+    if (i < D) {
+      const int base = 2 * D * D + (is_e ? 3 * D : 0) + i;
+      acc[(size_t)base * ld + walk] = m0;
+      acc[(size_t)(base + D) * ld + walk] = m1;
+      acc[(size_t)(base + 2 * D) * ld + walk] = m2;
+    }
+  }
+}
+
+namespace {
+
+__device__ __forceinline__ double rcp_nr_w(double x) {
+  const double r = __builtin_amdgcn_rcp(x);
+  return fma(r, fma(-x, r, 1.0), r);
+}
+__device__ __forceinline__ double rsq_nr_w(double x) {
+  const double r = __builtin_amdgcn_rsq(x);
+  return r * fma(-0.5 * x * r, r, 1.5);
+}
+
+// Sum over the wave's 64 lanes.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+}  // namespace
+
+// One wave per walk: the model algebra of walk_eigen_fp64 (F = X - P Z with
+// P = (sum X Z^T)(sum Z Z^T)^-1 on the projected rows; models 0..4 as in
+// construct_f_matrix, src/johansen_statistics.rs:102-197), then the
+// generalized eigenproblem of src/johansen_statistics.rs:24-47 as the
+// symmetric one of M = W^T W, W = L^-1 S10, S11 = L L^T.
+__global__ __launch_bounds__(64) void walk_eigen_wide(int model, int D, const double* __restrict__ acc, size_t ld,
+                                                      uint32_t T, uint32_t n_walks, double* __restrict__ out) {
+  __shared__ double Ls[kMaxNx][kMaxNx + 1];        // S11, then its Cholesky factor (lower)
+  __shared__ double Ws[kMaxNx][kMaxWideDim + 1];   // S10 (row i of X, column k of e), then W
+  __shared__ double Ms[kMaxWideDim][kMaxWideDim + 1];
+  __shared__ double P0[kMaxNx], P1[kMaxNx];
+  const int l = threadIdx.x;
+  const size_t walk = blockIdx.x;
+  if (walk >= n_walks) return;
+  const int NX = (model == 1 || model == 3) ? D + 1 : D;
+  const int NB = (model == 2 || model == 4) ? D - 1 : D;
+  const int NZ = model == 4 ? 2 : ((model == 2 || model == 3) ? 1 : 0);
+  auto A = [&](int e) -> double { return acc[(size_t)e * ld + walk]; };
+  auto GBB = [&](int i, int j) { return A(i * D + j); };
+  auto GEB = [&](int k, int j) { return A(D * D + k * D + j); };
+  auto MB = [&](int p, int i) { return A(2 * D * D + p * D + i); };
+  auto ME = [&](int p, int k) { return A(2 * D * D + (3 + p) * D + k); };
+
+  const double Tn = (double)T;
+  const double dt = 1.0 / Tn;
+  // closed forms of sum_t u_t^p, u_t = (t+1)/T, t = 0..T-1
+  const double c1 = (Tn + 1.0) * 0.5;
+  const double c2 = (Tn + 1.0) * (2.0 * Tn + 1.0) / (6.0 * Tn);
+  const double c3 = (Tn + 1.0) * (Tn + 1.0) / (4.0 * Tn);
+  const double c4 = (Tn + 1.0) * (2.0 * Tn + 1.0) * (3.0 * Tn * Tn + 3.0 * Tn - 1.0) / (30.0 * Tn * Tn * Tn);
+  auto XZ = [&](int x, int z) -> double {
+    if (x < NB) return MB(z, x);
+    if (model == 2 || model == 3) return c1 - 0.5 * Tn;  // sum y, y = u - 1/2
+    if (model == 4) return z == 0 ? c2 : c3;             // sum u^2, sum u^3
+    return 0.0;
+  };
+  auto projected = [&](int x) -> bool { return model == 4 || ((model == 2 || model == 3) && x < NB); };
+  auto XX = [&](int i, int j) -> double {
+    if (i < NB && j < NB) return GBB(i, j);
+    if (i >= NB && j >= NB) {
+      if (model == 1) return Tn;
+      if (model == 2 || model == 3) return c2 - c1 + 0.25 * Tn;  // sum (u - 1/2)^2
+      return c4;
+    }
+    const int bi = i < NB ? i : j;
+    if (model == 1) return MB(0, bi);
+    if (model == 2 || model == 3) return MB(1, bi) - 0.5 * MB(0, bi);  // sum b y
+    return MB(2, bi);                                                  // sum b u^2
+  };
+  auto EX = [&](int k, int j) -> double {
+    if (j < NB) return GEB(k, j);
+    if (model == 1) return ME(0, k);
+    if (model == 2 || model == 3) return ME(1, k) - 0.5 * ME(0, k);
+    return ME(2, k);
+  };
+  // ZZ^-1 in closed form: model 4: [[T, c1], [c1, c2]]^-1, det = (T^2 - 1)/12
+  double zi00 = 0.0, zi01 = 0.0, zi11 = 0.0;
+  if (NZ == 1) zi00 = dt;
+  if (NZ == 2) {
+    const double det = (Tn * Tn - 1.0) / 12.0;
+    zi00 = c2 / det;
+    zi01 = -c1 / det;
+    zi11 = Tn / det;
+  }
+  if (l < NX) {
+    double p0 = 0.0, p1 = 0.0;
+    if (NZ == 1) p0 = XZ(l, 0) * zi00;
+    if (NZ == 2) {
+      const double a0 = XZ(l, 0), a1 = XZ(l, 1);
+      p0 = a0 * zi00 + a1 * zi01;
+      p1 = a0 * zi01 + a1 * zi11;
+    }
+    P0[l] = p0;
+    P1[l] = p1;
+  }
+  __syncthreads();
+  auto proj = [&](int i, int j) -> double {  // XZ_i ZZ^-1 ZX_j on the projected rows
+    if (NZ == 0) return 0.0;
+    if (projected(i)) return NZ == 2 ? P0[i] * XZ(j, 0) + P1[i] * XZ(j, 1) : P0[i] * XZ(j, 0);
+    if (projected(j)) return NZ == 2 ? P0[j] * XZ(i, 0) + P1[j] * XZ(i, 1) : P0[j] * XZ(i, 0);
+    return 0.0;
+  };
+  // S11 (lower triangle) and S10 = S01raw^T: S01raw = eX - eZ ZZ^-1 ZX
+  for (int e = l; e < NX * NX; e += 64) {
+    const int i = e / NX, j = e - (e / NX) * NX;
+    if (j <= i) Ls[i][j] = (XX(i, j) - proj(i, j)) * dt;
+  }
+  for (int e = l; e < NX * D; e += 64) {
+    const int i = e / D, k = e - (e / D) * D;
+    double v = EX(k, i);
+    if (NZ > 0 && projected(i)) v -= NZ == 2 ? P0[i] * ME(0, k) + P1[i] * ME(1, k) : P0[i] * ME(0, k);
+    Ws[i][k] = v;
+  }
+  __syncthreads();
+
+  // Cholesky S11 = L L^T, one column at a time: every lane forms the pivot
+  // (the same value), lanes below it their entry of the column
+  bool pd = true;
+  for (int j = 0; j < NX; ++j) {
+    double s = Ls[j][j];
+    for (int k = 0; k < j; ++k) s -= Ls[j][k] * Ls[j][k];
+    pd = pd && (s > 0.0);
+    const double ljj = sqrt(s > 0.0 ? s : 1.0);
+    const double inv = 1.0 / ljj;
+    const int i = j + 1 + l;
+    double v = 0.0;
+    if (i < NX) {
+      v = Ls[i][j];
+      for (int k = 0; k < j; ++k) v -= Ls[i][k] * Ls[j][k];
+    }
+    __syncthreads();
+    if (l == 0) Ls[j][j] = ljj;
+    if (i < NX) Ls[i][j] = v * inv;
+    __syncthreads();
+  }
+
+  // W = L^-1 S10: lane k solves column k by forward substitution, in place
+  if (l < D) {
+    for (int i = 0; i < NX; ++i) {
+      double v = Ws[i][l];
+      for (int j = 0; j < i; ++j) v -= Ls[i][j] * Ws[j][l];
+      Ws[i][l] = v / Ls[i][i];
+    }
+  }
+  __syncthreads();
+  // M = W^T W (D x D, symmetric: both triangles)
+  for (int e = l; e < D * D; e += 64) {
+    const int k = e / D, m = e - (e / D) * D;
+    if (m < k) continue;
+    double v = 0.0;
+    for (int i = 0; i < NX; ++i) v = fma(Ws[i][k], Ws[i][m], v);
+    Ms[k][m] = v;
+    Ms[m][k] = v;
+  }
+  __syncthreads();
+
+  // cyclic threshold Jacobi on M: the rotation of pair (p, q) is formed by
+  // every lane from the same three entries; lane k updates row/column k
+  for (int sweep = 0; sweep < 40; ++sweep) {
+    double off = 0.0, dia = 0.0;
+    if (l < D) {
+      dia = Ms[l][l] * Ms[l][l];
+      for (int j = l + 1; j < D; ++j) off = fma(Ms[l][j], Ms[l][j], off);
+    }
+    off = wave_sum(off);
+    dia = wave_sum(dia);
+    if (!(off > 1e-34 * dia)) break;
+    for (int p = 0; p < D - 1; ++p) {
+      for (int q = p + 1; q < D; ++q) {
+        const double apq = Ms[p][q];
+        const double app = Ms[p][p], aqq = Ms[q][q];
+        double akp = 0.0, akq = 0.0;
+        if (l < D) {
+          akp = Ms[l][p];
+          akq = Ms[l][q];
+        }
+        __syncthreads();
+        if (apq * apq <= 1e-36 * fabs(app * aqq)) {
+          // below 1e-18 sqrt(|app aqq|): zeroing a_pq moves the eigenvalues by
+          // at most ~1e-18 of themselves
+          if (l == 0) {
+            Ms[p][q] = 0.0;
+            Ms[q][p] = 0.0;
+          }
+        } else {
+          const double theta = fmin(fmax((aqq - app) * rcp_nr_w(2.0 * apq), -1e150), 1e150);
+          const double h = fma(theta, theta, 1.0);
+          double t = rcp_nr_w(fabs(theta) + h * rsq_nr_w(h));
+          if (theta < 0.0) t = -t;
+          const double c = rsq_nr_w(fma(t, t, 1.0));
+          const double s = t * c;
+          if (l < D && l != p && l != q) {
+            const double np = c * akp - s * akq;
+            const double nq = s * akp + c * akq;
+            Ms[l][p] = np;
+            Ms[p][l] = np;
+            Ms[l][q] = nq;
+            Ms[q][l] = nq;
+          }
+          if (l == 0) {
+            Ms[p][p] = app - t * apq;
+            Ms[q][q] = aqq + t * apq;
+            Ms[p][q] = 0.0;
+            Ms[q][p] = 0.0;
+          }
+        }
+        __syncthreads();
+      }
+    }
+  }
+
+  // |lambda| sorted descending (rank of each by lane), NX - D structural zeros
+  const double nan = __longlong_as_double(0x7FF8000000000000LL);
+  double* o = out + walk * (size_t)NX;
+  if (l < D) {
+    const double ev = fabs(Ms[l][l]);
+    int rank = 0;
+    for (int j = 0; j < D; ++j) {
+      const double x = fabs(Ms[j][j]);
+      rank += (x > ev || (x == ev && j < l)) ? 1 : 0;
+    }
+    o[rank] = pd ? ev : nan;
+  }
+  if (l == 0 && NX > D) o[D] = pd ? 0.0 : nan;
+}
+
+hipError_t launch_walk_moments_wide(int d, const uint32_t* seeds, const double* zin, uint32_t rng_stream, uint32_t T,
+                                    uint32_t n, double* acc, size_t ld, hipStream_t st) {
+  if (d <= kMaxDim || d > kMaxWideDim) return hipErrorInvalidValue;
+  if (zin != nullptr)
+    hipLaunchKernelGGL(walk_moments_wide<true>, dim3(n), dim3(64), 0, st, seeds, zin, rng_stream, T, n, d, acc, ld);
+  else
+    hipLaunchKernelGGL(walk_moments_wide<false>, dim3(n), dim3(64), 0, st, seeds, zin, rng_stream, T, n, d, acc, ld);
+  return hipGetLastError();
+}
+
+hipError_t launch_walk_eigen_wide(int model, int d, const double* acc, size_t ld, uint32_t T, uint32_t n, double* out,
+                                  hipStream_t st) {
+  if (d <= kMaxDim || d > kMaxWideDim || model < 0 || model > 4) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(walk_eigen_wide, dim3(n), dim3(64), 0, st, model, d, acc, ld, T, n, out);
+  return hipGetLastError();
+}
+
+}  // namespace jne

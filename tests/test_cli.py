@@ -48,7 +48,7 @@ def test_version():
     (["--model", ","], "Error: 模型列表不可為空\n"),
     (["--threads", "100000"], None),
     # the engine's limits, checked before any device work
-    (["--dim", "13"], "Error: dimension 13 is above the supported maximum of 12\n"),
+    (["--dim", "33"], "Error: dimension 33 is above the supported maximum of 32\n"),
     (["--steps", "1,000,000,000,000"], "Error: steps must be between 2 and 1,073,741,824\n"),
     (["--steps", "1"], "Error: steps must be between 2 and 1,073,741,824\n"),
     (["--runs", "4,294,967,296"], "Error: runs must be at most 4,294,967,295 (seeds are u32)\n"),

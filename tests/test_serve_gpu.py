@@ -83,4 +83,4 @@ def test_wait_timeout_and_unknown_id(jne, server):
     with pytest.raises(jne.JneError):
         server.wait(i)
     with pytest.raises(jne.JneError):
-        server.submit(4, 13, 100, 10, 1)
+        server.submit(4, 33, 100, 10, 1)

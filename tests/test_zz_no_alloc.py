@@ -42,6 +42,15 @@ def test_noalloc_equals_the_allocating_entry_point(engine, jne, prec):
     assert np.isfinite(a).all() and np.array_equal(a, b)
 
 
+def test_noalloc_wide_dim(engine, jne):
+    """Above dim 12 (the runtime-dim FP64 path, no M scratch): the capacity
+    formula without the M term, and the same results as the allocating call."""
+    assert jne.noalloc_max_walks(32) == (64 << 20) // 8 // (2 * 32 * 32 + 6 * 32)
+    seeds = np.arange(1, 1001, dtype=np.uint32)
+    a, b = run_both(engine, 4, 20, 200, seeds, jne.PRECISIONS["auto"])
+    assert np.isfinite(a).all() and np.array_equal(a, b)
+
+
 def test_noalloc_small_dim_and_capacity(engine, jne):
     cap = jne.noalloc_max_walks(1)
     assert cap == (64 << 20) // 8 // 10
