@@ -446,12 +446,18 @@ def config1_full(pkg, eng, threads, host, model=0, dim=2, T=1000, runs=100_000):
         dat_bytes = os.path.getsize(path)
         fs, fe, *_ = pkg.read_dat(path)
         same = bool(np.array_equal(fe, gpu[fs.astype(np.int64) - 1]))
-    rel = np.abs(gpu - cpu) / (np.abs(cpu) + 1e-13 * cpu.max(axis=1, keepdims=True))
+    # the FP64 parity bar, |d| <= 1e-9 |rho| + 1e-13 rho_max: a ratio <= 1 meets it.  The smallest
+    # eigenvalue of some walks is ~1e-10 rho_max, where the problem's own conditioning leaves
+    # ~1e-7 relative between any two FP64 solvers (LAPACK dggev and the C oracle included)
+    d = np.abs(gpu - cpu)
+    cmax = cpu.max(axis=1, keepdims=True)
+    bar = d / (1e-9 * np.abs(cpu) + 1e-13 * cmax)
     return {"workload": f"Johansen model {model}, dim {dim}, T={T:,} steps, {runs:,} runs (seeds 1..{runs:,}), "
                         f"FP64 path", "cpu_port_walks_per_s": runs / cpu_s, "cpu_s": cpu_s, "cpu_threads": threads,
             "cpu_host": host.get("model_name"), "gpu_batch_walks_per_s": runs / gpu_s, "gpu_batch_s": gpu_s,
             "gpu_streamed_dat_s": sim_s, "gpu_streamed_walks_per_s": runs / sim_s, "dat_file_bytes": dat_bytes,
-            "dat_equals_batch": same, "gpu_vs_cpu_max_rel_diff": float(rel.max())}
+            "dat_equals_batch": same, "gpu_vs_cpu_max_normwise_diff": float((d / cmax).max()),
+            "gpu_vs_cpu_max_diff_over_fp64_bar": float(bar.max())}
 
 
 def main() -> int:

@@ -37,6 +37,15 @@
 // LDS row, as both the A and the B operand of X X^T.  Each lane keeps its
 // share of the FP64 Gram in registers; it goes to LDS once, after the last
 // segment, for the output.
+//
+// At dims 1, 2, 3, 4 and 6 one walk would leave most of the 32 x 32 tile and
+// of the lanes idle, so a wave carries W = 12 / dim walks as the 12
+// components of one walk of the kernel: walk w's components are rows
+// w dim .. w dim + dim - 1 of each block, drawn from its own Philox stream,
+// and the output keeps only the blocks on the diagonal (the products across
+// walks are formed and dropped).  Every quantity a walk's output depends on is
+// formed from that walk's own rows, so its results are those of a wave of its
+// own.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -64,6 +73,11 @@
 // unroll of the loop over a segment's 4 MFMA chunks
 #ifndef JNE_MFMA_CHUNK_UNROLL
 #define JNE_MFMA_CHUNK_UNROLL 4
+#endif
+// dims 1, 2, 3, 4 and 6: 12 / dim walks per wave, as the components of one
+// 12-component walk of the kernel (0: one walk per wave at every dim)
+#ifndef JNE_MFMA_PACK
+#define JNE_MFMA_PACK 1
 #endif
 
 namespace jne {
@@ -287,11 +301,14 @@ __device__ __forceinline__ uint32_t xcd_walk(uint32_t b, uint32_t n) {
   return x * q + (x < r ? x : r) + i;
 }
 
-template <int D, int PREC, bool FROM_NORMALS>
+// D: components of the wave (W walks of D / W each)
+template <int D, int PREC, bool FROM_NORMALS, int W = 1>
 __global__ __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(PREC == JNE_PREC_FP32_KAHAN ? 2 : JNE_MFMA_WAVES)))
 void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restrict__ zin, uint32_t rng_stream,
                        uint32_t T, uint32_t n_walks, double* __restrict__ acc, size_t ld) {
+  constexpr int DW = D / W;  // components of one walk
+  static_assert(D % W == 0 && (W == 1 || MfmaShape<D>::CALLS % 64 == 0), "packed walks need whole iterations");
   using Sh = MfmaShape<D>;
   constexpr int ZP = Sh::ZP, NG = Sh::NG, SP = Sh::SP, BS = Sh::BS;
   constexpr bool kKahan = PREC == JNE_PREC_FP32_KAHAN;
@@ -326,11 +343,19 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
   const int l = threadIdx.x;
   const int row = l & 31;  // also the accumulator column
   const int h = l >> 5;
-  if (blockIdx.x >= n_walks) return;  // whole wave exits together
-  const size_t walk = xcd_walk(blockIdx.x, n_walks);
+  const uint32_t n_groups = (n_walks + W - 1) / W;
+  if (blockIdx.x >= n_groups) return;  // whole wave exits together
+  const size_t walk = (size_t)xcd_walk(blockIdx.x, n_groups) * W;  // the wave's first walk
   uint32_t seed = 0;
-  if constexpr (!FROM_NORMALS) seed = seeds[walk];
-  const double* zw = FROM_NORMALS ? zin + walk * (size_t)T * D : nullptr;
+  if constexpr (!FROM_NORMALS && W == 1) seed = seeds[walk];
+  const double* zw = FROM_NORMALS ? zin + walk * (size_t)T * DW : nullptr;
+  // packed walks: a segment's Philox call c belongs to walk c / (16 DW) (a
+  // walk draws 16 DW calls per segment); the walks' seeds, in LDS
+  __shared__ uint32_t wseed[W];
+  if constexpr (W > 1 && !FROM_NORMALS) {
+    if (l < W) wseed[l] = seeds[walk + l < n_walks ? walk + l : n_walks - 1];
+    __syncthreads();
+  }
 
   const bool is_z = row < D;
   const bool is_c = row >= kRowC && row < kRowC + D;
@@ -450,6 +475,31 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
         pos.template advance<64>();
       }
       drawn += 64u * (uint32_t)nit;
+    } else if constexpr (W > 1) {
+      // call c: local call cl of walk w, whose normals 4 cl .. 4 cl + 3 are
+      // (step, component) of that walk, component i -> row w DW + i
+#pragma unroll CallUnroll<D, PREC, FROM_NORMALS>::value
+      for (int j = 0; j < Sh::IT; ++j) {
+        const int c = l + 64 * j;
+        const int w = c / (16 * DW), cl = c - w * (16 * DW);
+        float v[4];
+        if constexpr (!FROM_NORMALS) normals4((uint64_t)ts * DW / 4 + cl, wseed[w], rng_stream, v);
+        const CallPos<DW> p(cl);
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+          int tl, i;
+          p.at(jj, tl, i);
+          const uint32_t t = ts + tl;
+          float z;
+          if constexpr (FROM_NORMALS) {
+            const size_t wk = walk + w < n_walks ? (size_t)w : 0;
+            z = t < T ? (float)zw[wk * (size_t)T * DW + (size_t)t * DW + i] : 0.0f;
+          } else {
+            z = (!tail || t < T) ? v[jj] : 0.0f;
+          }
+          zbuf[w * DW + i][tl] = z;
+        }
+      }
     } else {
     CallPos<D> pos = pos0;
 #pragma unroll CallUnroll<D, PREC, FROM_NORMALS>::value
@@ -671,26 +721,46 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
     if (e < D * D) {
       const int i = e / D, j = e % D;
       const int lo = i < j ? i : j, hi = i < j ? j : i;  // one triangle: exactly symmetric
-      acc[(size_t)e * ld + walk] = bbA[q] + s2 * gram[hi * kGramPitch + kRowC + lo];
-      acc[(size_t)(D * D + e) * ld + walk] = ebA[q] + s2 * gram[j * kGramPitch + i];
+      const double vbb = bbA[q] + s2 * gram[hi * kGramPitch + kRowC + lo];
+      const double veb = ebA[q] + s2 * gram[j * kGramPitch + i];
+      if constexpr (W == 1) {
+        acc[(size_t)e * ld + walk] = vbb;
+        acc[(size_t)(D * D + e) * ld + walk] = veb;
+      } else {
+        // packed walks: the blocks on the diagonal, in each walk's own layout
+        const int w = i / DW, iw = i - w * DW, jw = j - w * DW;
+        if (j / DW == w && walk + w < n_walks) {
+          acc[(size_t)(iw * DW + jw) * ld + walk + w] = vbb;
+          acc[(size_t)(DW * DW + iw * DW + jw) * ld + walk + w] = veb;
+        }
+      }
     }
   }
   if (lx < 3 * D) {
     const int xp = lx / D, xi = lx % D;
-    acc[(size_t)(2 * D * D + xp * D + xi) * ld + walk] = mb[lx] + s * gram[xi * kGramPitch + kRowDet + xp];
-    acc[(size_t)(2 * D * D + (3 + xp) * D + xi) * ld + walk] = s * gram[kGramZ + 3 * xi + xp];
+    const double vb = mb[lx] + s * gram[xi * kGramPitch + kRowDet + xp];
+    const double ve = s * gram[kGramZ + 3 * xi + xp];
+    const int w = xi / DW, iw = xi - w * DW;
+    if (walk + w < n_walks) {
+      acc[(size_t)(2 * DW * DW + xp * DW + iw) * ld + walk + w] = vb;
+      acc[(size_t)(2 * DW * DW + (3 + xp) * DW + iw) * ld + walk + w] = ve;
+    }
   }
 }
 
 template <int D, int PREC>
 static hipError_t launch_mfma_dp(const uint32_t* seeds, const double* zin, uint32_t rng_stream, uint32_t T,
                                  uint32_t n, double* acc, size_t ld, hipStream_t st) {
+  // dims dividing 12: 12 / D walks per wave as one 12-component walk
+  constexpr int W = (JNE_MFMA_PACK && 12 % D == 0) ? 12 / D : 1;
+  constexpr int DS = D * W;
+  const unsigned grid = (n + W - 1) / W;
   if (zin != nullptr)
-    hipLaunchKernelGGL((walk_moments_mfma<D, PREC, true>), dim3(n), dim3(64), 0, st, seeds, zin, rng_stream, T, n,
-                       acc, ld);
+    hipLaunchKernelGGL((walk_moments_mfma<DS, PREC, true, W>), dim3(grid), dim3(64), 0, st, seeds, zin, rng_stream, T,
+                       n, acc, ld);
   else
-    hipLaunchKernelGGL((walk_moments_mfma<D, PREC, false>), dim3(n), dim3(64), 0, st, seeds, zin, rng_stream, T, n,
-                       acc, ld);
+    hipLaunchKernelGGL((walk_moments_mfma<DS, PREC, false, W>), dim3(grid), dim3(64), 0, st, seeds, zin, rng_stream,
+                       T, n, acc, ld);
   return hipGetLastError();
 }
 

@@ -130,6 +130,24 @@ def test_mixed_batch_invariance(engine, jne):
     assert np.array_equal(full[57], engine.eigenvalues_by_seed(model, dim, steps, seeds[57:58], precision=p)[0])
 
 
+@pytest.mark.parametrize("prec", ["mfma_fp16x2", "fp32_kahan", "mfma_bf16x3"])
+@pytest.mark.parametrize("dim", [1, 2, 3, 4, 6])
+def test_mixed_packed_walks_are_independent(engine, jne, prec, dim):
+    """Dims 1, 2, 3, 4 and 6 carry 12 / dim walks per wave: each walk's result is
+    that of the walk alone, whatever its neighbours and wherever the batch ends
+    (a batch of 25 leaves a partly filled last wave at every packing)."""
+    p = jne.PRECISIONS[prec]
+    model, steps = 4, 333
+    seeds = np.arange(1, 26, dtype=np.uint32)
+    full = engine.eigenvalues_by_seed(model, dim, steps, seeds, precision=p)
+    for k in (0, 5, 11, 12, 24):
+        assert np.array_equal(full[k], engine.eigenvalues_by_seed(model, dim, steps, seeds[k:k + 1], precision=p)[0]), k
+    perm = np.random.default_rng(dim).permutation(len(seeds))
+    assert np.array_equal(full[perm], engine.eigenvalues_by_seed(model, dim, steps, seeds[perm], precision=p))
+    z = engine.walk_normals(dim, steps, seeds).astype(np.float64)
+    assert np.array_equal(full, engine.eigenvalues_from_normals(model, z, precision=p))
+
+
 def test_mixed_deterministic_and_sorted(engine, jne):
     p = jne.PRECISIONS["mfma_fp16x2"]
     seeds = np.arange(1, 65, dtype=np.uint32)
