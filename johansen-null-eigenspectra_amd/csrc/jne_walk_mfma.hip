@@ -96,9 +96,11 @@ constexpr int kRowDet = 24;    // rows 24, 25, 26: 1, k, k(k-1)/2
 //   column c in [12, 24) (the c block): rows 0..26  at (c - 12) * 28 + r
 //   column c < 12 (the z block):        rows 24..26 at 336 + 3 c + (r - 24)
 // The c-block pitch of 28 keeps every group of 4 rows 16-byte aligned.
+//   symmetric form only, column c < 12: rows 12..23 at 372 + 12 c + (r - 12)
 constexpr int kGramPitch = 28;
 constexpr int kGramZ = 12 * kGramPitch;
-constexpr int kGramSize = kGramZ + 12 * 3;
+constexpr int kGramT = kGramZ + 12 * 3;
+constexpr int kGramSize = kGramT + 12 * 12;
 
 template <int D>
 struct MfmaShape {
@@ -145,12 +147,27 @@ __device__ __forceinline__ double uniform_f64(double x) {
 
 // MFMA of one 16-step chunk.  x[8] are this lane's 8 row values (steps 8h + k,
 // or 2k + h for the interleaved FP32 form).
+//
+// Symmetric form (kSymmetric): of the cross products of the split pieces only
+// one of each transposed pair is formed -- acc2 gets hi lo^T but not
+// lo hi^T = (hi lo^T)^T -- and the wave's FP64 Gram accumulates
+// F = acc + 2 acc2 for rows 0..23.  The Gram is then (F + F^T) / 2, formed once
+// per walk from the two transposed entries, which restores
+// hi hi^T + hi lo^T + lo hi^T: one MFMA per chunk fewer.  Rows 24..26 (the
+// constants 1, k, k(k-1)/2 with k < 64, at most 1,953) are exact in fp16, so
+// their lo is zero, acc + acc2 is already their full product and the
+// per-segment sums read it directly.  In bf16 (8 significant bits) k(k-1)/2 is
+// not exact, so the bf16 forms keep both cross products.
+#ifndef JNE_MFMA_SYM
+#define JNE_MFMA_SYM 1
+#endif
 template <int PREC>
 struct ChunkMfma;
 
 template <>
 struct ChunkMfma<JNE_PREC_MFMA_FP16X2> {
   static constexpr bool kInterleaved = false;
+  static constexpr bool kSymmetric = JNE_MFMA_SYM;
   __device__ __forceinline__ static void run(const float (&x)[8], floatx16& acc, floatx16& acc2) {
     half8 hi, lo;
 #if JNE_MFMA_SPLIT_ROUND
@@ -177,9 +194,13 @@ struct ChunkMfma<JNE_PREC_MFMA_FP16X2> {
 #endif
     acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(hi, hi, acc, 0, 0, 0);
     acc2 = __builtin_amdgcn_mfma_f32_32x32x16_f16(hi, lo, acc2, 0, 0, 0);
-    acc2 = __builtin_amdgcn_mfma_f32_32x32x16_f16(lo, hi, acc2, 0, 0, 0);
+    if constexpr (!kSymmetric) acc2 = __builtin_amdgcn_mfma_f32_32x32x16_f16(lo, hi, acc2, 0, 0, 0);
 #if JNE_MFMA_LOLO
-    acc2 = __builtin_amdgcn_mfma_f32_32x32x16_f16(lo, lo, acc2, 0, 0, 0);
+    // lo lo^T is symmetric: in the symmetric form it joins hi hi^T
+    if constexpr (kSymmetric)
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(lo, lo, acc, 0, 0, 0);
+    else
+      acc2 = __builtin_amdgcn_mfma_f32_32x32x16_f16(lo, lo, acc2, 0, 0, 0);
 #endif
   }
 };
@@ -187,6 +208,7 @@ struct ChunkMfma<JNE_PREC_MFMA_FP16X2> {
 template <>
 struct ChunkMfma<JNE_PREC_MFMA_BF16X2> {
   static constexpr bool kInterleaved = false;
+  static constexpr bool kSymmetric = false;
   __device__ __forceinline__ static void run(const float (&x)[8], floatx16& acc, floatx16& acc2) {
     bf16x8 hi, lo;
 #pragma unroll
@@ -204,6 +226,7 @@ struct ChunkMfma<JNE_PREC_MFMA_BF16X2> {
 template <>
 struct ChunkMfma<JNE_PREC_MFMA_BF16X3> {
   static constexpr bool kInterleaved = false;
+  static constexpr bool kSymmetric = false;
   __device__ __forceinline__ static void run(const float (&x)[8], floatx16& acc, floatx16& acc2) {
     bf16x8 hi, mi, lo;
 #pragma unroll
@@ -227,6 +250,7 @@ struct ChunkMfma<JNE_PREC_MFMA_BF16X3> {
 template <>
 struct ChunkMfma<JNE_PREC_MFMA_FP32> {
   static constexpr bool kInterleaved = true;
+  static constexpr bool kSymmetric = false;
   __device__ __forceinline__ static void run(const float (&x)[8], floatx16& acc, floatx16&) {
 #pragma unroll
     for (int m = 0; m < 8; ++m) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(x[m], x[m], acc, 0, 0, 0);
@@ -313,6 +337,7 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
   constexpr int ZP = Sh::ZP, NG = Sh::NG, SP = Sh::SP, BS = Sh::BS;
   constexpr bool kKahan = PREC == JNE_PREC_FP32_KAHAN;
   using Op = ChunkMfma<kKahan ? JNE_PREC_MFMA_FP32 : PREC>;
+  constexpr bool kSym = !kKahan && Op::kSymmetric;
   using KS = KahanShape<D>;
 
   // Row r of X lives in LDS row r: the segment's normals z (rows 0..D-1), its
@@ -438,6 +463,10 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
     constexpr bool tail = decltype(tail_tag)::value;
     const uint32_t ts = r * kSegSteps;
     // ---- normals of the segment -> LDS
+    // Wave priority: low while drawing (long runs of independent VALU work),
+    // high from the scan to the flush, whose LDS round trips and MFMA
+    // dependences leave gaps that the other waves' drawing then fills.
+    __builtin_amdgcn_s_setprio(0);
     if constexpr (kCarry) {
       // calls [r CALLS, drawn) came with the previous segment's last iteration
       const int pending = (int)(drawn - r * (uint32_t)Sh::CALLS);
@@ -525,6 +554,7 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
     }
     __syncthreads();
 
+    __builtin_amdgcn_s_setprio(2);
     // ---- scan, part 1: lane (si, sp) sums its block of component si in FP64:
     // the partial sums give the segment's eps, which drives the FP64 state,
     // and the offsets of the c rows in the blocks after it
@@ -623,7 +653,8 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
         Op::run(x, accm, accc);
       }
 #pragma unroll
-      for (int v = 0; v < 16; ++v) f[v] = (double)(accm[v] + accc[v]);
+      for (int v = 0; v < 16; ++v)
+        f[v] = (double)((kSym && v < 12) ? __builtin_fmaf(2.0f, accc[v], accm[v]) : accm[v] + accc[v]);
     }
 
     // ---- segment end
@@ -694,13 +725,28 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
 
   // the Gram entries the output reads, into LDS over the row buffers (their
   // last reads came before the last segment's barriers): c lanes rows 0..23 of
-  // their column, the lower half also rows 24..26; z lanes rows 24..26
+  // their column, the lower half also rows 24..26; z lanes rows 24..26, and in
+  // the symmetric form also rows 12..23 (the transposes of the z rows of the c
+  // columns)
   if (xc) {
     double2* const gcol = reinterpret_cast<double2*>(gram + (xrow_ - kRowC) * kGramPitch + 4 * xh);
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       gcol[4 * q] = make_double2(g[4 * q], g[4 * q + 1]);
       gcol[4 * q + 1] = make_double2(g[4 * q + 2], g[4 * q + 3]);
+    }
+  }
+  if constexpr (kSym) {
+    if (xz) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const int r0 = 8 * q + 4 * xh;  // this lane's rows r0..r0+3 of the column
+        if (r0 >= kRowC) {
+          double2* const gt = reinterpret_cast<double2*>(gram + kGramT + 12 * xrow_ + (r0 - kRowC));
+          gt[0] = make_double2(g[4 * q], g[4 * q + 1]);
+          gt[1] = make_double2(g[4 * q + 2], g[4 * q + 3]);
+        }
+      }
     }
   }
   if (xh == 0 && (xc || xz)) {
@@ -721,8 +767,12 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
     if (e < D * D) {
       const int i = e / D, j = e % D;
       const int lo = i < j ? i : j, hi = i < j ? j : i;  // one triangle: exactly symmetric
-      const double vbb = bbA[q] + s2 * gram[hi * kGramPitch + kRowC + lo];
-      const double veb = ebA[q] + s2 * gram[j * kGramPitch + i];
+      // symmetric form: the mean of the two transposed entries
+      const double gbb = kSym ? 0.5 * (gram[hi * kGramPitch + kRowC + lo] + gram[lo * kGramPitch + kRowC + hi])
+                              : gram[hi * kGramPitch + kRowC + lo];
+      const double geb = kSym ? 0.5 * (gram[j * kGramPitch + i] + gram[kGramT + 12 * i + j]) : gram[j * kGramPitch + i];
+      const double vbb = bbA[q] + s2 * gbb;
+      const double veb = ebA[q] + s2 * geb;
       if constexpr (W == 1) {
         acc[(size_t)e * ld + walk] = vbb;
         acc[(size_t)(D * D + e) * ld + walk] = veb;
