@@ -212,8 +212,9 @@ hipError_t timed_launch(jne_ctx* ctx, int kernel, uint64_t walks, hipStream_t st
 // JNE_PREC_AUTO: the MFMA pass where it measured faster than the FP64 pass
 // (walks/s, model 4, T = 100..10^4): dims 10-12 from 300 steps, dim 9 from 500
 // and dim 8 from 3,000; short walks carry the MFMA pass's per-walk setup.  At
-// dims 1-6 (12 / dim walks per wave at 1, 2, 3, 4 and 6) the two passes run
-// within 10% of each other, and the FP64 pass is kept for its accuracy.
+// dims 1-6 the FP64 pass keeps each walk in one lane's registers and runs
+// 1.2-1.45x the MFMA pass (which packs 12 / dim walks per wave at 1, 2, 3, 4
+// and 6); at dim 7 the two are even and the FP64 pass is kept for accuracy.
 bool auto_uses_mfma(int dim, uint32_t steps) {
   if (dim > JNE_MAX_MIXED_DIM) return false;
   if (dim >= 10) return steps >= 300;

@@ -12,12 +12,15 @@ constexpr int kMaxWideDim = 32;  // the runtime-dim FP64 path (jne_walk_wide.hip
 
 // Per-dim shape of the FP64 moment kernel: lanes per walk (L) and target
 // waves per SIMD (the register cap).  Measured on MI355X (model 4, T = 10^4).
+// Dims with L = 1 run the register-only kernel (its own wave targets); one
+// lane per walk at dims 7 and 8 would need a wave per SIMD and ran 11% and
+// 36% slower than the LDS-staged shapes.
 struct Fp64Shape {
   int lanes, waves;
 };
 #ifndef JNE_FP64_SHAPES
 #define JNE_FP64_SHAPES \
-  {1, 2}, {1, 2}, {1, 2}, {1, 2}, {5, 4}, {2, 3}, {7, 4}, {2, 2}, {9, 4}, {5, 3}, {4, 2}, {4, 2}
+  {1, 2}, {1, 2}, {1, 2}, {1, 2}, {1, 2}, {1, 2}, {7, 4}, {2, 2}, {9, 4}, {5, 3}, {4, 2}, {4, 2}
 #endif
 constexpr Fp64Shape kFp64Shapes[12] = {JNE_FP64_SHAPES};
 constexpr int lanes_for_dim(int d) { return kFp64Shapes[d - 1].lanes; }
@@ -51,6 +54,13 @@ struct WalkShape {
 // Step s and component i (n = s*D + i) of the normals n = 4c .. 4c+3 drawn by
 // Philox call c of a round.  One division where the position is created, then
 // constant strides: the loops over calls advance c by a compile-time step.
+// A compile-time flag as a type, for generic lambdas that take a branch-free
+// copy of a loop body (the tail form of a segment or batch).
+template <bool B>
+struct BoolTag {
+  static constexpr bool value = B;
+};
+
 template <int D>
 struct CallPos {
   int s, i;  // of normal 4c

@@ -27,6 +27,10 @@
 #ifndef JNE_FP64_UNROLL_CALLS
 #define JNE_FP64_UNROLL_CALLS 0
 #endif
+// dims with one lane per walk: the register-only kernel (0: the LDS-staged one)
+#ifndef JNE_FP64_REG
+#define JNE_FP64_REG 1
+#endif
 
 
 namespace jne {
@@ -228,6 +232,138 @@ void walk_moments_fp64(const uint32_t* __restrict__ seeds, const double* __restr
     }
 #pragma unroll
     for (int p = 0; p < 6; ++p) acc[(size_t)(2 * D * D + p * D + ci) * ld + walk] = mom[p][k].value();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Kernel 1, one lane per walk (the dims whose shape has L = 1): the same moments and the same
+// arithmetic, step by step, as walk_moments_fp64 with L = 1, with the
+// increments kept in registers instead of LDS.  A lane draws its walk's
+// Philox calls in batches of NB = 3 x lcm(dim, 4) / 4 calls (three
+// independent calls in flight) and consumes the batch's G = 4 NB / dim steps
+// from registers: no LDS, no barriers.
+// ---------------------------------------------------------------------------
+// target waves per SIMD of the register-only kernel by dim (the register cap:
+// 512 / waves VGPRs per lane); it runs the dims whose shape has one lane per walk
+#ifndef JNE_FP64_REG_WAVES
+#define JNE_FP64_REG_WAVES 4, 4, 3, 2, 2, 2, 2, 2, 2, 2, 2, 2
+#endif
+constexpr int kFp64RegWaves[12] = {JNE_FP64_REG_WAVES};
+
+template <int D>
+struct RegBatch {
+  static constexpr int NC = (4 / gcd_int(D, 4)) * D / 4;  // calls per whole-step group
+  static constexpr int NB = NC >= 3 ? NC : 3 * NC;         // calls per batch
+  static constexpr int G = 4 * NB / D;                     // steps per batch
+};
+
+template <int D, bool FROM_NORMALS>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kFp64RegWaves[D - 1], kFp64RegWaves[D - 1])))
+void walk_moments_fp64_reg(const uint32_t* __restrict__ seeds, const double* __restrict__ zin, uint32_t rng_stream,
+                           uint32_t T, uint32_t n_walks, double* __restrict__ acc, size_t ld) {
+  using Acc = GramAcc;
+  constexpr int H = D / 2;
+  constexpr int NB = RegBatch<D>::NB, G = RegBatch<D>::G;
+  const size_t walk = (size_t)blockIdx.x * 64 + threadIdx.x;
+  const bool valid = walk < n_walks;
+  const size_t wc = valid ? walk : (size_t)(n_walks - 1);
+  uint32_t seed = 0;
+  if constexpr (!FROM_NORMALS) seed = seeds[wc];
+  const double* zw = FROM_NORMALS ? zin + wc * (size_t)T * D : nullptr;
+
+  double b[D];
+  Acc geb[D][D];
+  Acc gbb[D][H + 1];
+  Acc mom[6][D];
+#pragma unroll
+  for (int m = 0; m < D; ++m) b[m] = 0.0;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+#pragma unroll
+    for (int m = 0; m < D; ++m) geb[k][m].zero();
+#pragma unroll
+    for (int m = 0; m <= H; ++m) gbb[k][m].zero();
+#pragma unroll
+    for (int p = 0; p < 6; ++p) mom[p][k].zero();
+  }
+
+  // src/johansen_statistics.rs:70 and src/rng_matrix.rs:137: sqrt(1/steps)
+  const double dt = 1.0 / (double)T;
+  const double sqrt_dt = sqrt(dt);
+
+  // one step of the walk from its scaled increments w[0..D) (the moment
+  // updates of walk_moments_fp64 with one lane per walk)
+  auto step = [&](const double* w, uint32_t t) {
+    const double u = (double)(t + 1) * dt;
+    const double u2 = u * u;
+    double bn[D], e[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      bn[k] = b[k] + w[k];  // B_{t+1} = B_t + z*sqrt(dt) (rng_matrix.rs:138-140)
+      e[k] = bn[k] - b[k];  // dbm = bm_current - bm_previous (johansen_statistics.rs:80-82)
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+#pragma unroll
+      for (int m = 0; m < D; ++m) geb[k][m].add_prod(e[k], b[m]);
+#pragma unroll
+      for (int dl = 0; dl <= H; ++dl) gbb[k][dl].add_prod(b[k], b[(k + dl) % D]);
+      mom[0][k].add(b[k]);
+      mom[1][k].add_prod(u, b[k]);
+      mom[2][k].add_prod(u2, b[k]);
+      mom[3][k].add(e[k]);
+      mom[4][k].add_prod(u, e[k]);
+      mom[5][k].add_prod(u2, e[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) b[k] = bn[k];
+  };
+
+  // a batch: G steps from t0; tail batches mask the steps at or beyond T
+  auto batch = [&](uint32_t t0, auto tail_tag) {
+    constexpr bool tail = decltype(tail_tag)::value;
+    double w[G * D];
+    if constexpr (!FROM_NORMALS) {
+      const uint64_t q0 = (uint64_t)t0 * D / 4;
+#pragma unroll
+      for (int c = 0; c < NB; ++c) {
+        float v[4];
+        normals4(q0 + c, seed, rng_stream, v);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) w[4 * c + j] = (double)v[j] * sqrt_dt;
+      }
+    } else {
+#pragma unroll
+      for (int n = 0; n < G * D; ++n) {
+        const uint32_t t = t0 + (uint32_t)(n / D);
+        w[n] = (!tail || t < T) ? zw[(size_t)t0 * D + n] * sqrt_dt : 0.0;
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < G; ++s) {
+      if (tail && t0 + s >= T) break;
+      step(&w[s * D], t0 + s);
+    }
+  };
+  const uint32_t full = T / G;
+  for (uint32_t q = 0; q < full; ++q) batch(q * G, BoolTag<false>{});
+  if (full * G < T) batch(full * G, BoolTag<true>{});
+
+  if (!valid) return;
+  // canonical layout, entry-major: [0, D^2) sum b b^T (full), [D^2, 2D^2)
+  // sum e b^T (row e, col b), then 6 moment vectors of length D.
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+#pragma unroll
+    for (int m = 0; m < D; ++m) acc[(size_t)(D * D + k * D + m) * ld + walk] = geb[k][m].value();
+#pragma unroll
+    for (int dl = 0; dl <= H; ++dl) {
+      const int cj = (k + dl) % D;
+      acc[(size_t)(k * D + cj) * ld + walk] = gbb[k][dl].value();
+      acc[(size_t)(cj * D + k) * ld + walk] = gbb[k][dl].value();
+    }
+#pragma unroll
+    for (int p = 0; p < 6; ++p) acc[(size_t)(2 * D * D + p * D + k) * ld + walk] = mom[p][k].value();
   }
 }
 
@@ -556,6 +692,17 @@ static hipError_t launch_moments_d(const uint32_t* seeds, const double* zin, uin
                                    uint32_t n, double* acc, size_t ld, hipStream_t st) {
   constexpr int GPW = WalkShape<D>::GPW;
   const unsigned grid = (unsigned)((n + GPW - 1) / GPW);
+#if JNE_FP64_REG
+  if constexpr (WalkShape<D>::L == 1) {
+    if (zin != nullptr)
+      hipLaunchKernelGGL((walk_moments_fp64_reg<D, true>), dim3(grid), dim3(64), 0, st, seeds, zin, rng_stream, T, n,
+                         acc, ld);
+    else
+      hipLaunchKernelGGL((walk_moments_fp64_reg<D, false>), dim3(grid), dim3(64), 0, st, seeds, zin, rng_stream, T, n,
+                         acc, ld);
+    return hipGetLastError();
+  }
+#endif
   if (zin != nullptr)
     hipLaunchKernelGGL((walk_moments_fp64<D, true>), dim3(grid), dim3(64), 0, st, seeds, zin, rng_stream, T, n, acc,
                        ld);

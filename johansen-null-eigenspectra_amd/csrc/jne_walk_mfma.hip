@@ -307,11 +307,6 @@ struct CallUnroll {
           : 1;
 };
 
-template <bool B>
-struct BoolTag {
-  static constexpr bool value = B;
-};
-
 // Workgroups are dealt to the 8 XCDs round-robin by index, and each XCD has
 // its own L2.  With one walk per workgroup, walks b and b+1 would run on
 // different XCDs and their 8-byte stores into one 32-byte sector of the
