@@ -13,8 +13,9 @@
 //                      (Philox calls dealt over the lanes, the stream of
 //                      jne_rng.h), lanes 0..dim-1 run the FP64 recurrence
 //                      B_{t+1} = B_t + z sqrt(dt) of their component, and every
-//                      lane adds its 8 x 4 tile of sum [b; e] b^T and one row of
-//                      the moment vectors, FP64 fma;
+//                      lane adds its tile of sum [b; e] b^T (components padded
+//                      to 16, 24 or 32) and one row of the moment vectors,
+//                      FP64 fma;
 //   walk_eigen_wide    model algebra (S11 and S01 as Schur complements on the
 //                      moments), Cholesky of S11, W = L^-1 S10, M = W^T W and
 //                      cyclic Jacobi on M, with lanes working on matrix
@@ -26,6 +27,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "jne_internal.h"
 #include "jne_rng.h"
 
@@ -35,7 +38,6 @@ namespace {
 
 constexpr int kPadDim = kMaxWideDim;            // components padded to 32
 constexpr int kRoundMax = 16;                   // most steps in one round (dim 13..16)
-constexpr int kYRow = 2 * kPadDim;              // LDS row of one step: [b (32) | e (32)]
 constexpr int kMaxNx = kMaxWideDim + 1;         // nF of models 1 and 3 at dim 32
 
 // Steps per round: a multiple of 4 (whole Philox calls for any dim) with at
@@ -44,16 +46,29 @@ __host__ __device__ constexpr int wide_round_steps(int d) { return ((256 / d) & 
 
 }  // namespace
 
-// One wave per walk (blockIdx.x).  Lane l owns rows 8(l/8)..8(l/8)+7 of the
-// stacked rows y = [b (32, zero-padded); e (32)] against columns
-// 4(l%8)..4(l%8)+3 of b: sum y b^T covers sum b b^T and sum e b^T; and the
-// moments sum y, sum u y, sum u^2 y of row l.
-template <bool FROM_NORMALS>
+// One wave per walk (blockIdx.x).  The components are padded to DP = 16, 24
+// or 32 (dims 13-16, 17-24, 25-32), and lane l owns an RB x CB tile of the
+// stacked rows y = [b (DP, zero-padded); e (DP)] against the columns b:
+// rows RB (l / NCB) .. + RB - 1, columns CB (l % NCB) .. + CB - 1, so that the
+// 64 tiles cover sum y b^T = [sum b b^T; sum e b^T] (DP = 16: 2 x 4 tiles,
+// 24: 3 x 6, 32: 8 x 4); lanes l < 2 DP also add sum y_l, sum u y_l and
+// sum u^2 y_l.
+template <int DP>
+struct WideTile {
+  static constexpr int CB = DP == 24 ? 6 : 4;             // columns per lane
+  static constexpr int NCB = DP / CB;                     // column blocks
+  static constexpr int RB = 2 * DP / (64 / NCB);          // rows per lane
+  static_assert((2 * DP / RB) * NCB == 64 && RB * (64 / NCB) == 2 * DP, "64 tiles");
+};
+
+template <int DP, bool FROM_NORMALS>
 __global__ __launch_bounds__(64) void walk_moments_wide(const uint32_t* __restrict__ seeds,
                                                         const double* __restrict__ zin, uint32_t rng_stream,
                                                         uint32_t T, uint32_t n_walks, int D,
                                                         double* __restrict__ acc, size_t ld) {
-  __shared__ __align__(16) double ys[kRoundMax][kYRow];  // per step of the round: b, then e
+  using Tl = WideTile<DP>;
+  constexpr int RB = Tl::RB, CB = Tl::CB, NCB = Tl::NCB;
+  __shared__ __align__(16) double ys[kRoundMax][2 * DP];  // per step of the round: b, then e
   __shared__ double wz[kRoundMax * kPadDim];              // the round's increments z sqrt(dt), t-major
   const int l = threadIdx.x;
   const size_t walk = blockIdx.x;
@@ -65,16 +80,17 @@ __global__ __launch_bounds__(64) void walk_moments_wide(const uint32_t* __restri
   const double* zw = FROM_NORMALS ? zin + walk * (size_t)T * D : nullptr;
 
   // padding rows and columns stay zero
-  for (int e = l; e < kRoundMax * kYRow; e += 64) (&ys[0][0])[e] = 0.0;
+  for (int e = l; e < kRoundMax * 2 * DP; e += 64) (&ys[0][0])[e] = 0.0;
   __syncthreads();
 
-  const int rb = l >> 3, cb = l & 7;
-  double g[8][4];
+  const int rb = l / NCB, cb = l % NCB;
+  double g[RB][CB];
 #pragma unroll
-  for (int r = 0; r < 8; ++r)
+  for (int r = 0; r < RB; ++r)
 #pragma unroll
-    for (int c = 0; c < 4; ++c) g[r][c] = 0.0;
-  double m0 = 0.0, m1 = 0.0, m2 = 0.0;  // sum y_l, sum u y_l, sum u^2 y_l
+    for (int c = 0; c < CB; ++c) g[r][c] = 0.0;
+  double m0 = 0.0, m1 = 0.0, m2 = 0.0;  // sum y_l, sum u y_l, sum u^2 y_l (lanes l < 2 DP)
+  const int lm = l < 2 * DP ? l : 0;
 
   // src/johansen_statistics.rs:70 and src/rng_matrix.rs:137: sqrt(1/steps)
   const double dt = 1.0 / (double)T;
@@ -108,33 +124,23 @@ __global__ __launch_bounds__(64) void walk_moments_wide(const uint32_t* __restri
       for (int s = 0; s < ns; ++s) {
         const double bn = b + wz[s * D + l];
         ys[s][l] = b;
-        ys[s][kPadDim + l] = bn - b;
+        ys[s][DP + l] = bn - b;
         b = bn;
       }
     }
     __syncthreads();
     for (int s = 0; s < ns; ++s) {
-      const double2* yr = reinterpret_cast<const double2*>(&ys[s][8 * rb]);
-      const double2* bc = reinterpret_cast<const double2*>(&ys[s][4 * cb]);
-      double y[8], bb[4];
+      double y[RB], bb[CB];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const double2 v = yr[k];
-        y[2 * k] = v.x;
-        y[2 * k + 1] = v.y;
-      }
+      for (int k = 0; k < RB; ++k) y[k] = ys[s][RB * rb + k];
 #pragma unroll
-      for (int k = 0; k < 2; ++k) {
-        const double2 v = bc[k];
-        bb[2 * k] = v.x;
-        bb[2 * k + 1] = v.y;
-      }
+      for (int k = 0; k < CB; ++k) bb[k] = ys[s][CB * cb + k];
 #pragma unroll
-      for (int r = 0; r < 8; ++r)
+      for (int r = 0; r < RB; ++r)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) g[r][c] = fma(y[r], bb[c], g[r][c]);
+        for (int c = 0; c < CB; ++c) g[r][c] = fma(y[r], bb[c], g[r][c]);
       const double u = (double)(t0 + (uint32_t)s + 1u) * dt;
-      const double yl = ys[s][l];
+      const double yl = ys[s][lm];
       m0 += yl;
       m1 = fma(u, yl, m1);
       m2 = fma(u * u, yl, m2);
@@ -145,20 +151,20 @@ __global__ __launch_bounds__(64) void walk_moments_wide(const uint32_t* __restri
   // canonical layout, entry-major: [0, D^2) sum b b^T, [D^2, 2D^2) sum e b^T
   // (row e, column b), then sum b, sum u b, sum u^2 b, sum e, sum u e, sum u^2 e
 #pragma unroll
-  for (int r = 0; r < 8; ++r) {
-    const int row = 8 * rb + r;
-    const bool is_e = row >= kPadDim;
-    const int i = is_e ? row - kPadDim : row;
+  for (int r = 0; r < RB; ++r) {
+    const int row = RB * rb + r;
+    const bool is_e = row >= DP;
+    const int i = is_e ? row - DP : row;
     if (i >= D) continue;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const int j = 4 * cb + c;
+    for (int c = 0; c < CB; ++c) {
+      const int j = CB * cb + c;
       if (j < D) acc[(size_t)((is_e ? D * D : 0) + i * D + j) * ld + walk] = g[r][c];
     }
   }
-  {
-    const bool is_e = l >= kPadDim;
-    const int i = is_e ? l - kPadDim : l;
+  if (l < 2 * DP) {
+    const bool is_e = l >= DP;
+    const int i = is_e ? l - DP : l;
     if (i < D) {
       const int base = 2 * D * D + (is_e ? 3 * D : 0) + i;
       acc[(size_t)base * ld + walk] = m0;
@@ -397,10 +403,21 @@ __global__ __launch_bounds__(64) void walk_eigen_wide(int model, int D, const do
 hipError_t launch_walk_moments_wide(int d, const uint32_t* seeds, const double* zin, uint32_t rng_stream, uint32_t T,
                                     uint32_t n, double* acc, size_t ld, hipStream_t st) {
   if (d <= kMaxDim || d > kMaxWideDim) return hipErrorInvalidValue;
-  if (zin != nullptr)
-    hipLaunchKernelGGL(walk_moments_wide<true>, dim3(n), dim3(64), 0, st, seeds, zin, rng_stream, T, n, d, acc, ld);
+  auto go = [&](auto dp_tag) {
+    constexpr int DP = decltype(dp_tag)::value;
+    if (zin != nullptr)
+      hipLaunchKernelGGL((walk_moments_wide<DP, true>), dim3(n), dim3(64), 0, st, seeds, zin, rng_stream, T, n, d, acc,
+                         ld);
+    else
+      hipLaunchKernelGGL((walk_moments_wide<DP, false>), dim3(n), dim3(64), 0, st, seeds, zin, rng_stream, T, n, d,
+                         acc, ld);
+  };
+  if (d <= 16)
+    go(std::integral_constant<int, 16>{});
+  else if (d <= 24)
+    go(std::integral_constant<int, 24>{});
   else
-    hipLaunchKernelGGL(walk_moments_wide<false>, dim3(n), dim3(64), 0, st, seeds, zin, rng_stream, T, n, d, acc, ld);
+    go(std::integral_constant<int, 32>{});
   return hipGetLastError();
 }
 
