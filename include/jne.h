@@ -47,8 +47,9 @@ extern "C" {
 /* The MFMA modes keep the state B and the per-64-step segment terms in FP64
  * and put only within-segment corrections on the matrix cores.  Both paths
  * draw the same normals, whose generation bounds them both; measured on
- * MI355X the MFMA pass is faster for dims 10-12 from 300 steps, dim 9 from
- * 500 steps and dim 8 from 3,000 steps, so JNE_PREC_AUTO runs MFMA_FP16X2
+ * MI355X the MFMA pass is faster for dim 12 from 200 steps, dims 10-11 from
+ * 300, dim 9 from 500, dim 8 from 1,000 and dim 7 from 3,000 steps, so
+ * JNE_PREC_AUTO runs MFMA_FP16X2
  * there and FP64 elsewhere (FP64 results meet the mixed-path tolerance
  * trivially). */
 

@@ -210,16 +210,18 @@ hipError_t timed_launch(jne_ctx* ctx, int kernel, uint64_t walks, hipStream_t st
 
 // Enqueue the walk path for n walks (device pointers) on stream st.
 // JNE_PREC_AUTO: the MFMA pass where it measured faster than the FP64 pass
-// (walks/s, model 4, T = 100..10^4): dims 10-12 from 300 steps, dim 9 from 500
-// and dim 8 from 3,000; short walks carry the MFMA pass's per-walk setup.  At
-// dims 1-6 the FP64 pass keeps each walk in one lane's registers and runs
-// 1.2-1.45x the MFMA pass (which packs 12 / dim walks per wave at 1, 2, 3, 4
-// and 6); at dim 7 the two are even and the FP64 pass is kept for accuracy.
+// (walks/s of the whole path, model 4, T = 100..10^4): dim 12 from 200 steps,
+// dims 10-11 from 300, dim 9 from 500, dim 8 from 1,000 and dim 7 from 3,000;
+// short walks carry the MFMA pass's per-walk setup.  At dims 1-6 the FP64 pass
+// keeps each walk in one lane's registers and runs 1.2-1.45x the MFMA pass
+// (which packs 12 / dim walks per wave at 1, 2, 3, 4 and 6).
 bool auto_uses_mfma(int dim, uint32_t steps) {
   if (dim > JNE_MAX_MIXED_DIM) return false;
+  if (dim == 12) return steps >= 200;
   if (dim >= 10) return steps >= 300;
   if (dim == 9) return steps >= 500;
-  if (dim == 8) return steps >= 3000;
+  if (dim == 8) return steps >= 1000;
+  if (dim == 7) return steps >= 3000;
   return false;
 }
 

@@ -169,13 +169,15 @@ def test_fp32_kahan_all_dims(engine, jne):
 
 
 def test_auto_precision_picks_the_faster_path(engine, jne):
-    """auto runs the MFMA pass for dims 10-12 from 300 steps, dim 9 from 500 and
-    dim 8 from 3,000 (the measured crossover), the FP64 pass elsewhere."""
+    """auto runs the MFMA pass for dim 12 from 200 steps, dims 10-11 from 300,
+    dim 9 from 500, dim 8 from 1,000 and dim 7 from 3,000 (the measured
+    crossover), the FP64 pass elsewhere."""
     seeds = np.arange(1, 33, dtype=np.uint32)
-    for dim, steps, same_as in [(12, 700, "mfma_fp16x2"), (11, 300, "mfma_fp16x2"), (12, 299, "fp64"),
-                                (10, 300, "mfma_fp16x2"), (9, 500, "mfma_fp16x2"), (10, 299, "fp64"), (9, 499, "fp64"),
-                                (8, 3000, "mfma_fp16x2"), (8, 2999, "fp64"), (7, 10000, "fp64"),
-                                (3, 700, "fp64")]:
+    for dim, steps, same_as in [(12, 200, "mfma_fp16x2"), (12, 199, "fp64"), (11, 300, "mfma_fp16x2"),
+                                (11, 299, "fp64"), (10, 300, "mfma_fp16x2"), (10, 299, "fp64"),
+                                (9, 500, "mfma_fp16x2"), (9, 499, "fp64"), (8, 1000, "mfma_fp16x2"), (8, 999, "fp64"),
+                                (7, 3000, "mfma_fp16x2"), (7, 2999, "fp64"), (6, 10000, "fp64"), (3, 700, "fp64"),
+                                (13, 10000, "fp64")]:
         a = engine.eigenvalues_by_seed(4, dim, steps, seeds, precision=jne.PRECISIONS["auto"])
         b = engine.eigenvalues_by_seed(4, dim, steps, seeds, precision=jne.PRECISIONS[same_as])
         assert np.array_equal(a, b), (dim, steps)
