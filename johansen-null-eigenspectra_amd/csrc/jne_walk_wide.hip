@@ -38,7 +38,6 @@ namespace {
 
 constexpr int kPadDim = kMaxWideDim;            // components padded to 32
 constexpr int kRoundMax = 16;                   // most steps in one round (dim 13..16)
-constexpr int kMaxNx = kMaxWideDim + 1;         // nF of models 1 and 3 at dim 32
 
 // Steps per round: a multiple of 4 (whole Philox calls for any dim) with at
 // most 64 calls, so each lane draws at most one call per round.
@@ -194,6 +193,20 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 }  // namespace
 
+// A row-major matrix in LDS with a runtime pitch: m[i][j].
+struct LdsMat {
+  double* p;
+  int pitch;
+  __device__ __forceinline__ double* operator[](int i) const { return p + i * pitch; }
+};
+
+// LDS bytes of walk_eigen_wide for (model, dim): L (NX x (NX + 1)), W (NX x
+// (D + 1)) and P0, P1 (NX each), NX = D + 1 for models 1 and 3, else D.
+size_t wide_eigen_lds_bytes(int model, int D) {
+  const size_t nx = (model == 1 || model == 3) ? (size_t)D + 1 : (size_t)D;
+  return sizeof(double) * (nx * (nx + 1) + nx * ((size_t)D + 1) + 2 * nx);
+}
+
 // One wave per walk: the model algebra of walk_eigen_fp64 (F = X - P Z with
 // P = (sum X Z^T)(sum Z Z^T)^-1 on the projected rows; models 0..4 as in
 // construct_f_matrix, src/johansen_statistics.rs:102-197), then the
@@ -201,14 +214,20 @@ __device__ __forceinline__ double wave_sum(double v) {
 // symmetric one of M = W^T W, W = L^-1 S10, S11 = L L^T.
 __global__ __launch_bounds__(64) void walk_eigen_wide(int model, int D, const double* __restrict__ acc, size_t ld,
                                                       uint32_t T, uint32_t n_walks, double* __restrict__ out) {
-  __shared__ double Ls[kMaxNx][kMaxNx + 1];        // S11, then its Cholesky factor (lower)
-  __shared__ double Ws[kMaxNx][kMaxWideDim + 1];   // S10 (row i of X, column k of e), then W
-  __shared__ double Ms[kMaxWideDim][kMaxWideDim + 1];
-  __shared__ double P0[kMaxNx], P1[kMaxNx];
+  // LDS sized by the launch for this (model, dim) (wide_eigen_lds_bytes):
+  // S11 and then its Cholesky factor L (NX x NX, pitch NX + 1); S10 and then
+  // W (NX x D, pitch D + 1); the projections P0, P1.  Once W is formed the
+  // bytes of L hold M (D x D, pitch D + 1), which needs only W.
+  extern __shared__ double wide_lds[];
+  const int NX = (model == 1 || model == 3) ? D + 1 : D;
+  const LdsMat Ls{wide_lds, NX + 1};
+  const LdsMat Ws{wide_lds + NX * (NX + 1), D + 1};
+  const LdsMat Ms{wide_lds, D + 1};
+  double* const P0 = wide_lds + NX * (NX + 1) + NX * (D + 1);
+  double* const P1 = P0 + NX;
   const int l = threadIdx.x;
   const size_t walk = blockIdx.x;
   if (walk >= n_walks) return;
-  const int NX = (model == 1 || model == 3) ? D + 1 : D;
   const int NB = (model == 2 || model == 4) ? D - 1 : D;
   const int NZ = model == 4 ? 2 : ((model == 2 || model == 3) ? 1 : 0);
   auto A = [&](int e) -> double { return acc[(size_t)e * ld + walk]; };
@@ -424,7 +443,8 @@ hipError_t launch_walk_moments_wide(int d, const uint32_t* seeds, const double* 
 hipError_t launch_walk_eigen_wide(int model, int d, const double* acc, size_t ld, uint32_t T, uint32_t n, double* out,
                                   hipStream_t st) {
   if (d <= kMaxDim || d > kMaxWideDim || model < 0 || model > 4) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(walk_eigen_wide, dim3(n), dim3(64), 0, st, model, d, acc, ld, T, n, out);
+  hipLaunchKernelGGL(walk_eigen_wide, dim3(n), dim3(64), wide_eigen_lds_bytes(model, d), st, model, d, acc, ld, T, n,
+                     out);
   return hipGetLastError();
 }
 
