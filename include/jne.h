@@ -109,6 +109,20 @@ int jne_eigenvalues_by_seed_device(jne_ctx *ctx, int model, int dim, uint32_t st
                                    const uint32_t *d_seeds, size_t n, int precision, double *d_out,
                                    void *hip_stream);
 
+/* Several models of the same walks: out d_outs[k] (n x jne_num_eigenvalues
+ * (models[k], dim) doubles, device memory) gets model models[k]'s eigenvalues
+ * of the n walks of d_seeds, k < n_models (1..5).  A walk's moments do not
+ * depend on the model, so the moment pass runs once per walk and only the
+ * model algebra and eigen solve run per model: the result for each model is
+ * bit for bit that of jne_eigenvalues_by_seed_device for it.  The reference
+ * recomputes the same walks for every model of its sweep
+ * (src/main.rs:85-127 runs each model's seeds 1..runs through
+ * calculate_eigenvalues, src/johansen_statistics.rs:59-85); this replaces
+ * those repeats.  Stream and scratch as jne_eigenvalues_by_seed_device. */
+int jne_eigenvalues_by_seed_models_device(jne_ctx *ctx, const int *models, int n_models, int dim, uint32_t steps,
+                                          uint32_t rng_stream, const uint32_t *d_seeds, size_t n, int precision,
+                                          double *const *d_outs, void *hip_stream);
+
 /* No counterpart in the reference: for callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc).  The same as jne_eigenvalues_by_seed_device, with the
@@ -268,6 +282,17 @@ int jne_dat_statistics(const char *path, uint64_t runs, int statistic, double *o
  * ------------------------------------------------------------------------ */
 int jne_run_simulation(jne_ctx *ctx, int model, int dim, uint32_t steps, uint64_t runs, int precision,
                        const char *path, int quiet);
+
+/* The models[0..n_models) (1..5) of one dim, to paths[k]: the reference's
+ * sweep runs them one after another over the same seeds (src/main.rs:109-127,
+ * each through run_model_simulation).  With quiet = 0 that is what happens
+ * here, message for message.  Quiet, the models whose files need the same
+ * seeds (all of them in a fresh sweep) run together: one moment pass per
+ * walk and one eigen solve per model (jne_eigenvalues_by_seed_models_device),
+ * each file's records the same bytes as its own jne_run_simulation would
+ * write.  Incompatible files are replaced as in jne_run_simulation. */
+int jne_run_simulation_models(jne_ctx *ctx, const int *models, int n_models, int dim, uint32_t steps, uint64_t runs,
+                              int precision, const char *const *paths, int quiet);
 
 /* ------------------------------------------------------------------------
  * Query server.  No counterpart in the reference, which only runs batch

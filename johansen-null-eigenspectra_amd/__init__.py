@@ -134,6 +134,11 @@ def load_library() -> ctypes.CDLL:
         L.jne_eigenvalues_by_seed_device.restype = ctypes.c_int
         L.jne_eigenvalues_by_seed_device_noalloc.argtypes = L.jne_eigenvalues_by_seed_device.argtypes
         L.jne_eigenvalues_by_seed_device_noalloc.restype = ctypes.c_int
+        L.jne_eigenvalues_by_seed_models_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                                            ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32,
+                                                            ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
+                                                            ctypes.c_void_p, ctypes.c_void_p]
+        L.jne_eigenvalues_by_seed_models_device.restype = ctypes.c_int
         L.jne_eigenvalues_from_normals.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
                                                    ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p]
         L.jne_eigenvalues_from_normals.restype = ctypes.c_int
@@ -190,6 +195,10 @@ def load_library() -> ctypes.CDLL:
         L.jne_run_simulation.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
                                          ctypes.c_uint64, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
         L.jne_run_simulation.restype = ctypes.c_int
+        L.jne_run_simulation_models.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p,
+                                                ctypes.c_int]
+        L.jne_run_simulation_models.restype = ctypes.c_int
         L.jne_server_create.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_int, c_void_pp]
         L.jne_server_create.restype = ctypes.c_int
         L.jne_server_destroy.argtypes = [ctypes.c_void_p]
@@ -326,6 +335,26 @@ class Engine:
         self._check(self._lib.jne_eigenvalues_by_seed_device(
             self._ctx, int(model), int(dim), int(steps), int(rng_stream), ctypes.c_void_p(d_seeds), int(n),
             int(precision), ctypes.c_void_p(d_out), _stream_arg(hip_stream)))
+
+    def eigenvalues_by_seed_models_device(self, models, dim: int, steps: int, d_seeds: int, n: int, d_outs,
+                                          rng_stream: int = 0, precision: int = PREC_FP64, hip_stream=None):
+        """Several models of the same walks (device pointers): d_outs[k] gets model models[k]'s
+        eigenvalues; one moment pass per walk, one eigen solve per model."""
+        m = np.ascontiguousarray(models, dtype=np.int32)
+        outs = (ctypes.c_void_p * len(m))(*[ctypes.c_void_p(int(p)) for p in d_outs])
+        self._check(self._lib.jne_eigenvalues_by_seed_models_device(
+            self._ctx, m.ctypes.data, len(m), int(dim), int(steps), int(rng_stream), ctypes.c_void_p(d_seeds),
+            int(n), int(precision), outs, _stream_arg(hip_stream)))
+
+    def run_simulation_models(self, models, dim: int, steps: int, runs: int, paths, precision: int = PREC_FP64,
+                              quiet: bool = True):
+        """jne_run_simulation_models: the models of one dim to their .dat files (quiet: the models
+        needing the same seeds share one moment pass per walk)."""
+        m = np.ascontiguousarray(models, dtype=np.int32)
+        enc = [os.fsencode(str(p)) for p in paths]
+        cp = (ctypes.c_char_p * len(enc))(*enc)
+        _host_check(self._lib.jne_run_simulation_models(self._ctx, m.ctypes.data, len(m), int(dim), int(steps),
+                                                        int(runs), int(precision), cp, 1 if quiet else 0))
 
     def eigenvalues_by_seed_device_noalloc(self, model: int, dim: int, steps: int, d_seeds: int, n: int, d_out: int,
                                            rng_stream: int = 0, precision: int = PREC_FP64, hip_stream=None):

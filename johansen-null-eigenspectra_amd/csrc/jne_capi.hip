@@ -227,13 +227,14 @@ bool auto_uses_mfma(int dim, uint32_t steps) {
 
 // Enqueues the walk kernels for n walks on st, in chunks of at most `chunk`
 // walks over the scratch `acc` (chunk * moment_count(dim) doubles) and `mbuf`
-// (chunk * eigen_m_count(dim) doubles).  timed: record the launches when
+// (chunk * eigen_m_count(dim) doubles): per chunk one moment pass, then the
+// eigen solve of each of the n_models models into its output (the moments of
+// a walk do not depend on the model).  timed: record the launches when
 // profiling is enabled.
-int launch_walks(jne_ctx* ctx, int model, int dim, uint32_t steps, uint32_t rng_stream, const uint32_t* d_seeds,
-                 const double* d_z, size_t n, int precision, double* d_out, hipStream_t st, double* acc,
-                 double* mbuf, size_t chunk, bool timed) {
+int launch_walks(jne_ctx* ctx, const int* models, int n_models, int dim, uint32_t steps, uint32_t rng_stream,
+                 const uint32_t* d_seeds, const double* d_z, size_t n, int precision, double* const* d_outs,
+                 hipStream_t st, double* acc, double* mbuf, size_t chunk, bool timed) {
   if (precision == JNE_PREC_AUTO) precision = auto_uses_mfma(dim, steps) ? JNE_PREC_MFMA_FP16X2 : JNE_PREC_FP64;
-  const int nf = jne::num_eigenvalues(model, dim);
   for (size_t off = 0; off < n; off += chunk) {
     const uint32_t cn = (uint32_t)std::min(chunk, n - off);
     const uint32_t* s = d_seeds ? d_seeds + off : nullptr;
@@ -243,23 +244,29 @@ int launch_walks(jne_ctx* ctx, int model, int dim, uint32_t steps, uint32_t rng_
       if (precision == JNE_PREC_FP64) return jne::launch_walk_moments_fp64(dim, s, z, rng_stream, steps, cn, acc, cn, st);
       return jne::launch_walk_moments_mfma(dim, precision, s, z, rng_stream, steps, cn, acc, cn, st);
     };
-    auto eigen = [&] {
-      if (dim > jne::kMaxDim) return jne::launch_walk_eigen_wide(model, dim, acc, cn, steps, cn, d_out + off * nf, st);
-      return jne::launch_walk_eigen_fp64(model, dim, acc, cn, steps, cn, d_out + off * nf, mbuf, st);
-    };
-    if (timed) {
+    if (timed)
       JNE_HIP_CHECK(ctx, timed_launch(ctx, JNE_KERNEL_MOMENTS, cn, st, moments));
-      JNE_HIP_CHECK(ctx, timed_launch(ctx, JNE_KERNEL_EIGEN, cn, st, eigen));
-    } else {
+    else
       JNE_HIP_CHECK(ctx, moments());
-      JNE_HIP_CHECK(ctx, eigen());
+    for (int k = 0; k < n_models; ++k) {
+      const int model = models[k];
+      double* const out = d_outs[k] + off * (size_t)jne::num_eigenvalues(model, dim);
+      auto eigen = [&] {
+        if (dim > jne::kMaxDim) return jne::launch_walk_eigen_wide(model, dim, acc, cn, steps, cn, out, st);
+        return jne::launch_walk_eigen_fp64(model, dim, acc, cn, steps, cn, out, mbuf, st);
+      };
+      if (timed)
+        JNE_HIP_CHECK(ctx, timed_launch(ctx, JNE_KERNEL_EIGEN, cn, st, eigen));
+      else
+        JNE_HIP_CHECK(ctx, eigen());
     }
   }
   return JNE_OK;
 }
 
-int run_walks(jne_ctx* ctx, int model, int dim, uint32_t steps, uint32_t rng_stream, const uint32_t* d_seeds,
-              const double* d_z, size_t n, int precision, double* d_out, hipStream_t st) {
+int run_walks(jne_ctx* ctx, const int* models, int n_models, int dim, uint32_t steps, uint32_t rng_stream,
+              const uint32_t* d_seeds, const double* d_z, size_t n, int precision, double* const* d_outs,
+              hipStream_t st) {
   const size_t mc = (size_t)jne::moment_count(dim);
   // at most 2^18 walks per launch, and at the wide dims about 1 GiB of moments
   const size_t chunk = std::min(n, std::min(kChunkWalks, std::max<size_t>(65536, (size_t(1) << 27) / mc)));
@@ -270,8 +277,8 @@ int run_walks(jne_ctx* ctx, int model, int dim, uint32_t steps, uint32_t rng_str
   if (!ctx->scratch_free) JNE_HIP_CHECK(ctx, hipEventCreateWithFlags(&ctx->scratch_free, hipEventDisableTiming));
   if (ctx->scratch_stream && ctx->scratch_stream != st)
     JNE_HIP_CHECK(ctx, hipStreamWaitEvent(st, ctx->scratch_free, 0));
-  if ((rc = launch_walks(ctx, model, dim, steps, rng_stream, d_seeds, d_z, n, precision, d_out, st, ctx->d_acc,
-                         ctx->d_m, chunk, true)))
+  if ((rc = launch_walks(ctx, models, n_models, dim, steps, rng_stream, d_seeds, d_z, n, precision, d_outs, st,
+                         ctx->d_acc, ctx->d_m, chunk, true)))
     return rc;
   JNE_HIP_CHECK(ctx, hipEventRecord(ctx->scratch_free, st));
   ctx->scratch_stream = st;
@@ -382,8 +389,9 @@ int jne_eigenvalues_by_seed(jne_ctx* ctx, int model, int dim, uint32_t steps, ui
   if ((rc = ensure(ctx, &ctx->d_in, &ctx->in_bytes, n * sizeof(uint32_t)))) return rc;
   if ((rc = ensure(ctx, &ctx->d_out, &ctx->out_bytes, n * nf * sizeof(double)))) return rc;
   JNE_HIP_CHECK(ctx, hipMemcpyAsync(ctx->d_in, seeds, n * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
-  if ((rc = run_walks(ctx, model, dim, steps, rng_stream, (const uint32_t*)ctx->d_in, nullptr, n, precision,
-                      (double*)ctx->d_out, ctx->stream)))
+  double* const outs[1] = {(double*)ctx->d_out};
+  if ((rc = run_walks(ctx, &model, 1, dim, steps, rng_stream, (const uint32_t*)ctx->d_in, nullptr, n, precision, outs,
+                      ctx->stream)))
     return rc;
   return finish_host_out(ctx, out, n * nf);
 }
@@ -400,7 +408,28 @@ int jne_eigenvalues_by_seed_device(jne_ctx* ctx, int model, int dim, uint32_t st
   if (!d_seeds || !d_out) return set_err(ctx, JNE_ERR_INVALID_ARG, "d_seeds/d_out must not be NULL");
   JNE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   hipStream_t st = stream_of(ctx, hip_stream);
-  return run_walks(ctx, model, dim, steps, rng_stream, d_seeds, nullptr, n, precision, d_out, st);
+  double* const outs[1] = {d_out};
+  return run_walks(ctx, &model, 1, dim, steps, rng_stream, d_seeds, nullptr, n, precision, outs, st);
+}
+
+int jne_eigenvalues_by_seed_models_device(jne_ctx* ctx, const int* models, int n_models, int dim, uint32_t steps,
+                                          uint32_t rng_stream, const uint32_t* d_seeds, size_t n, int precision,
+                                          double* const* d_outs, void* hip_stream) {
+  if (!ctx) return JNE_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->err.clear();
+  if (!models || !d_outs || n_models < 1 || n_models > 5)
+    return set_err(ctx, JNE_ERR_INVALID_ARG, "models/d_outs must not be NULL and n_models must be in 1..5");
+  for (int k = 0; k < n_models; ++k) {
+    int rc = check_args(ctx, models[k], dim, steps, precision);
+    if (rc) return rc;
+    if (n > 0 && !d_outs[k]) return set_err(ctx, JNE_ERR_INVALID_ARG, "d_outs[%d] must not be NULL", k);
+  }
+  if (n == 0) return JNE_OK;
+  if (!d_seeds) return set_err(ctx, JNE_ERR_INVALID_ARG, "d_seeds must not be NULL");
+  JNE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  return run_walks(ctx, models, n_models, dim, steps, rng_stream, d_seeds, nullptr, n, precision, d_outs,
+                   stream_of(ctx, hip_stream));
 }
 
 int jne_eigenvalues_by_seed_device_noalloc(jne_ctx* ctx, int model, int dim, uint32_t steps, uint32_t rng_stream,
@@ -424,7 +453,8 @@ int jne_eigenvalues_by_seed_device_noalloc(jne_ctx* ctx, int model, int dim, uin
   if (!ctx->noalloc_scratch) return set_err(ctx, JNE_ERR_HIP, "the no-alloc scratch has no device address");
   JNE_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   double* const acc = ctx->noalloc_scratch;
-  return launch_walks(ctx, model, dim, steps, rng_stream, d_seeds, nullptr, n, precision, d_out,
+  double* const outs[1] = {d_out};
+  return launch_walks(ctx, &model, 1, dim, steps, rng_stream, d_seeds, nullptr, n, precision, outs,
                       stream_of(ctx, hip_stream), acc, acc + mc * n, n, false);
 }
 
@@ -443,7 +473,8 @@ int jne_eigenvalues_from_normals(jne_ctx* ctx, int model, int dim, uint32_t step
   if ((rc = ensure(ctx, &ctx->d_in, &ctx->in_bytes, zcount * sizeof(double)))) return rc;
   if ((rc = ensure(ctx, &ctx->d_out, &ctx->out_bytes, n * nf * sizeof(double)))) return rc;
   JNE_HIP_CHECK(ctx, hipMemcpyAsync(ctx->d_in, z, zcount * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-  if ((rc = run_walks(ctx, model, dim, steps, 0, nullptr, (const double*)ctx->d_in, n, precision, (double*)ctx->d_out,
+  double* const outs[1] = {(double*)ctx->d_out};
+  if ((rc = run_walks(ctx, &model, 1, dim, steps, 0, nullptr, (const double*)ctx->d_in, n, precision, outs,
                       ctx->stream)))
     return rc;
   return finish_host_out(ctx, out, n * nf);

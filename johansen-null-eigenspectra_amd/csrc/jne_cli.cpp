@@ -351,6 +351,26 @@ int main(int argc, char** argv) {
              commas(a.runs).c_str());
     }
     fflush(stdout);
+    bool fuse = q && models.size() > 1 && models.size() <= 5;
+    for (size_t i = 0; fuse && i < models.size(); ++i)
+      for (size_t j = i + 1; j < models.size(); ++j)
+        if (models[i] == models[j]) fuse = false;
+    if (fuse) {
+      // quiet: the models of this dim share one moment pass per walk
+      // (jne_run_simulation_models); the files are those of the loop below
+      std::vector<std::string> paths;
+      std::vector<const char*> cpaths;
+      for (int model : models) paths.push_back(dat_filename("data", model, (int)dim, a.steps, true));
+      for (const std::string& p : paths) cpaths.push_back(p.c_str());
+      const int rc = jne_run_simulation_models(ctx, models.data(), (int)models.size(), (int)dim, (uint32_t)a.steps,
+                                               a.runs, a.precision, cpaths.data(), 1);
+      if (rc != JNE_OK) {
+        fprintf(stderr, "Error: %s (dim %llu): %s\n", rc == JNE_ERR_NUMERIC ? "numeric failure" : "failed",
+                (unsigned long long)dim, jne_host_last_error());
+        status = 1;
+      }
+      continue;
+    }
     for (int model : models) {
       const std::string path = dat_filename("data", model, (int)dim, a.steps, true);
       const int rc = jne_run_simulation(ctx, model, (int)dim, (uint32_t)a.steps, a.runs, a.precision, path.c_str(),
