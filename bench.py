@@ -416,6 +416,45 @@ def cpu_baseline_block(model, dim, T, threads, head, fp64_head, k, host):
     return block
 
 
+def config3_sweep(pkg, eng, runs=100_000, T=10_000, dims=range(1, 13), models=(0, 1, 2, 3, 4)):
+    """The reference's production sweep (models 0-4 x dims 1-12, T = 10^4) at `runs` runs per
+    pair, streamed to .dat files: the quiet CLI (the models of a dim share one moment pass per walk,
+    jne_run_simulation_models) against the same sweep model by model (one jne_run_simulation-
+    equivalent call per pair), with the 60 files compared byte for byte."""
+    import filecmp
+    import subprocess
+    import tempfile
+
+    cli = os.path.join(ROOT, "johansen-null-eigenspectra_amd", "bin", "johansen-null-eigenspectra")
+    with tempfile.TemporaryDirectory() as tmp:
+        fused_dir, single_dir = os.path.join(tmp, "fused"), os.path.join(tmp, "single")
+        os.makedirs(fused_dir)
+        os.makedirs(os.path.join(single_dir, "data"))
+        t0 = time.perf_counter()
+        subprocess.run([cli, "--runs", f"{runs:,}", "--steps", f"{T:,}", "--dim-start", str(min(dims)),
+                        "--dim-end", str(max(dims)), "--precision", "auto", "--quiet"], cwd=fused_dir, check=True,
+                       capture_output=True, timeout=600)
+        fused_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for dim in dims:
+            for m in models:
+                eng.run_simulation_models([m], dim, T, runs,
+                                          [os.path.join(single_dir, "data", f"eigenvalues_model{m}_dim{dim}_steps{T}.dat")],
+                                          precision=pkg.PRECISIONS["auto"])
+        single_s = time.perf_counter() - t0
+        names = sorted(os.listdir(os.path.join(fused_dir, "data")))
+        same = len(names) == len(dims) * len(models) and all(
+            filecmp.cmp(os.path.join(fused_dir, "data", nm), os.path.join(single_dir, "data", nm), shallow=False)
+            for nm in names)
+        nbytes = sum(os.path.getsize(os.path.join(fused_dir, "data", nm)) for nm in names)
+    walks = runs * len(dims) * len(models)
+    return {"workload": f"production sweep: models 0-4 x dims {min(dims)}-{max(dims)}, T={T:,}, {runs:,} runs per "
+                        f"pair, auto precision, streamed to .dat", "walks": walks, "files": len(names),
+            "bytes": nbytes, "fused_cli_s": fused_s, "fused_walks_per_s": walks / fused_s,
+            "model_by_model_s": single_s, "model_by_model_walks_per_s": walks / single_s,
+            "files_identical": same}
+
+
 def config1_full(pkg, eng, threads, host, model=0, dim=2, T=1000, runs=100_000):
     """The reference's CPU-native configuration in full (model 0, dim 2,
     T = 1,000, 100,000 runs; src/data_storage/parallel_compute.rs:14-41): the
@@ -476,6 +515,8 @@ def main() -> int:
     ap.add_argument("--no-rng-rate", action="store_true", help="skip the generator-alone rate (bin/jne_rng_rate)")
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="0 = every CPU of this job's share of the host (affinity, cgroup quota, OMP_NUM_THREADS)")
+    ap.add_argument("--no-config3", action="store_true",
+                    help="skip the production sweep sample (models 0-4 x dims 1-12, T=10^4, 10^5 runs per pair)")
     ap.add_argument("--no-config1", action="store_true",
                     help="skip the plain-CPU configuration (model 0, dim 2, T=1,000, 100,000 runs) on CPU and GPU")
     ap.add_argument("--serve-rates", default="10,30,100,300,1000",
@@ -607,6 +648,8 @@ def main() -> int:
     threads = args.cpu_threads or host["cpu_share"]
     if not args.no_config1:
         line["config1"] = config1_full(pkg, eng, threads, host)
+    if not args.no_config3:
+        line["config3"] = config3_sweep(pkg, eng)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline_block(model, dim, T, threads, head, fp64_head, k, host)
     print(json.dumps(line), flush=True)
