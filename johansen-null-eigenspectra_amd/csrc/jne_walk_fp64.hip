@@ -239,9 +239,10 @@ void walk_moments_fp64(const uint32_t* __restrict__ seeds, const double* __restr
 // Kernel 1, one lane per walk (the dims whose shape has L = 1): the same moments and the same
 // arithmetic, step by step, as walk_moments_fp64 with L = 1, with the
 // increments kept in registers instead of LDS.  A lane draws its walk's
-// Philox calls in batches of NB = 3 x lcm(dim, 4) / 4 calls (three
-// independent calls in flight) and consumes the batch's G = 4 NB / dim steps
-// from registers: no LDS, no barriers.
+// Philox calls in batches of NB calls (kFp64RegCalls: three independent
+// calls in flight at dims 1-4 and 6, the five of a whole-step group at dim 5)
+// and consumes the batch's G = 4 NB / dim steps from registers: no LDS, no
+// barriers.
 // ---------------------------------------------------------------------------
 // target waves per SIMD of the register-only kernel by dim (the register cap:
 // 512 / waves VGPRs per lane); it runs the dims whose shape has one lane per walk
@@ -250,11 +251,19 @@ void walk_moments_fp64(const uint32_t* __restrict__ seeds, const double* __restr
 #endif
 constexpr int kFp64RegWaves[12] = {JNE_FP64_REG_WAVES};
 
+// Philox calls per batch by dim (a multiple of the calls of one whole-step
+// group: 1, 1, 3, 1, 5, 3 at dims 1..6)
+#ifndef JNE_FP64_REG_CALLS
+#define JNE_FP64_REG_CALLS 3, 3, 3, 3, 5, 3, 3, 3, 3, 3, 3, 3
+#endif
+constexpr int kFp64RegCalls[12] = {JNE_FP64_REG_CALLS};
+
 template <int D>
 struct RegBatch {
   static constexpr int NC = (4 / gcd_int(D, 4)) * D / 4;  // calls per whole-step group
-  static constexpr int NB = NC >= 3 ? NC : 3 * NC;         // calls per batch
+  static constexpr int NB = kFp64RegCalls[D - 1];         // calls per batch
   static constexpr int G = 4 * NB / D;                     // steps per batch
+  static_assert(NB % NC == 0, "a batch must be whole steps");
 };
 
 template <int D, bool FROM_NORMALS>
