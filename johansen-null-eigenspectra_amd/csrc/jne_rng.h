@@ -51,34 +51,40 @@ __device__ __forceinline__ u32x4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_
   return u32x4{c0, c1, c2, c3};
 }
 
-// ln(u) for u in [2^-33, 1]: u = m * 2^e with m in [sqrt(1/2), sqrt(2)),
-// ln(m) = f + f^2 Q(f), f = m - 1 (exact).  The split comes from
-// v_frexp_mant/v_frexp_exp (mant in [1/2, 1)): m = mant, or 2 mant below
-// sqrt(1/2) with e - 1 -- the same (m, e) as taking the exponent field.
-__device__ __forceinline__ float ln_poly(float u) {
+// 2 ln(u) for u in [2^-33, 1].  The logarithm: u = m * 2^e with m in (C, 2C],
+// C = 0x1.6a09e6p-1 (sqrt(1/2) rounded down), and ln(u) = L = e ln2_hi +
+// (e ln2_lo + lnm), lnm = f + f^2 Q(f), f = m - 1 (exact; Horner in float32
+// with fixed coefficients).  Positive normal floats order as their bit
+// patterns and adding k 2^23 to a pattern scales the value by 2^k, so u in
+// (2C 2^(e-1), 2C 2^e] gives -e = floor((bits(2C) - bits(u)) / 2^23) (one
+// subtraction, one shift) and m = u 2^-e (v_ldexp_f32): the (m, e) of the
+// split by frexp (mant in [1/2, 1); m = 2 mant, e = exp - 1 when mant <= C)
+// for every u of the range.  The return value is 2L computed at twice the
+// scale: h = 2f = 2m - 2, Horner in h with coefficients c_k 2^(-k-1) (= Q/2,
+// every partial sum exactly 2^(-k-1) times the one in f), 2 lnm = h + h^2 Q/2
+// and the ln 2 terms doubled, -e entering with their signs turned.  Scaling
+// by a power of two commutes with rounding (no partial result is subnormal),
+// so this is exactly 2L, +0 at u = 1, and -(2L) is bit for bit -2 * L.
+__device__ __forceinline__ float two_ln(float u) {
 #pragma clang fp contract(off)
-  const float mant = __builtin_amdgcn_frexp_mantf(u);
-  const int ex = __builtin_amdgcn_frexp_expf(u);
-  // 0x1.6a09e6p-1 is sqrt(1/2) rounded down: m = 2 mant whenever 2 mant is at
-  // most sqrt(2) rounded down, as with the exponent-field split
-  const bool lo = mant <= 0x1.6a09e6p-1f;
-  const float m = lo ? mant + mant : mant;
-  const int e = lo ? ex - 1 : ex;
-  const float f = m - 1.0f;
-  float q = 0x1.15425cp-4f;
-  q = __builtin_fmaf(q, f, -0x1.d962f2p-4f);
-  q = __builtin_fmaf(q, f, 0x1.e3d33ap-4f);
-  q = __builtin_fmaf(q, f, -0x1.fcc1aep-4f);
-  q = __builtin_fmaf(q, f, 0x1.237a56p-3f);
-  q = __builtin_fmaf(q, f, -0x1.5559e6p-3f);
-  q = __builtin_fmaf(q, f, 0x1.99a13p-3f);
-  q = __builtin_fmaf(q, f, -0x1.000008p-2f);
-  q = __builtin_fmaf(q, f, 0x1.55554ep-2f);
-  q = __builtin_fmaf(q, f, -0x1p-1f);
-  const float f2 = f * f;
-  const float lnm = __builtin_fmaf(f2, q, f);
-  const float ef = (float)e;
-  return __builtin_fmaf(ef, 0x1.62e43p-1f, __builtin_fmaf(ef, -0x1.05c61p-29f, lnm));
+  const int32_t ub = (int32_t)__float_as_uint(u);
+  const int32_t ne = ((int32_t)0x3FB504F3 - ub) >> 23;  // 0x3FB504F3 = bits(2C); ne = -e
+  const float m = __builtin_amdgcn_ldexpf(u, ne);  // exact: m is a normal float
+  const float h = __builtin_fmaf(m, 2.0f, -2.0f);  // 2f, exact
+  float q = 0x1.15425cp-14f;
+  q = __builtin_fmaf(q, h, -0x1.d962f2p-13f);
+  q = __builtin_fmaf(q, h, 0x1.e3d33ap-12f);
+  q = __builtin_fmaf(q, h, -0x1.fcc1aep-11f);
+  q = __builtin_fmaf(q, h, 0x1.237a56p-9f);
+  q = __builtin_fmaf(q, h, -0x1.5559e6p-8f);
+  q = __builtin_fmaf(q, h, 0x1.99a13p-7f);
+  q = __builtin_fmaf(q, h, -0x1.000008p-5f);
+  q = __builtin_fmaf(q, h, 0x1.55554ep-4f);
+  q = __builtin_fmaf(q, h, -0x1p-2f);
+  const float h2 = h * h;
+  const float lnm2 = __builtin_fmaf(h2, q, h);
+  const float nef = (float)ne;
+  return __builtin_fmaf(nef, -0x1.62e43p+0f, __builtin_fmaf(nef, 0x1.05c61p-28f, lnm2));
 }
 
 // Correctly rounded sqrt(y) for y = +-0 or y in [2^-96, 2^126]: v_sqrt_f32
@@ -101,9 +107,9 @@ __device__ __forceinline__ float sqrt_cr(float y) {
 __device__ __forceinline__ void normal_pair(uint32_t wa, uint32_t wb, float& z0, float& z1) {
 #pragma clang fp contract(off)
   const float u1 = __builtin_fmaf((float)wa, 0x1p-32f, 0x1p-33f);  // (0, 1]
-  // -2 ln(u1) is +-0 or in [5.9e-8, 46.1]
-  const float r = sqrt_cr(-2.0f * ln_poly(u1));
-  const uint32_t q = wb >> 30;
+  // -2 ln(u1) is -0 (u1 = 1) or in [5.9e-8, 46.1]; the negation is a source
+  // modifier of the square root's instructions
+  const float r = sqrt_cr(-two_ln(u1));
   const int32_t gi = (int32_t)(wb & 0x3FFFFFFFu) - (int32_t)0x20000000;
   const float g = (float)gi * 0x1p-30f;  // [-1/2, 1/2)
   const float x = g * g;
@@ -118,16 +124,24 @@ __device__ __forceinline__ void normal_pair(uint32_t wa, uint32_t wb, float& z0,
   cp = __builtin_fmaf(cp, x, 0x1.6f5a4ap-3f);
   cp = __builtin_fmaf(cp, x, -0x1.bea5b6p-1f);
   cp = __builtin_fmaf(cp, x, 0x1.6a09e6p-1f);
-  const float s = g * sp;   // sin(pi g/2)/sqrt2
-  const float cq = cp - s;  // cos(pi/4 + pi g/2)
-  const float sq = cp + s;  // sin(pi/4 + pi g/2)
-  const bool odd = (q & 1u) != 0u;
-  const float a = odd ? sq : cq;
-  const float b = odd ? cq : sq;
-  const uint32_t cs = ((q ^ (q >> 1)) & 1u) << 31;
-  const uint32_t ss = (q >> 1) << 31;
-  z0 = __uint_as_float(__float_as_uint(r * a) ^ cs);
-  z1 = __uint_as_float(__float_as_uint(r * b) ^ ss);
+  const float s = g * sp;  // sin(pi g/2)/sqrt2
+  // (cp - s, cp + s) = (cos, sin)(pi/4 + pi g/2), turned by q = wb >> 30
+  // quarter turns: q = 0: (cp - s, cp + s), 1: (-(cp + s), cp - s),
+  // 2: (-(cp - s), -(cp + s)), 3: (cp + s, -(cp - s)).  The factor pair
+  // (cp + s', cp - s'), s' = s negated when bit 30 of wb is clear, picks the
+  // entries; the signs go on r as sign flips of a product: bit 31 flips both
+  // outputs, bit 30 flips z0 once more.  Sign flips and x + (-y) = x - y are
+  // exact, so this is bit for bit the select-and-flip of the rotation.
+  // v_bitop3_b32 truth tables over (x, y, 0x80000000): 0x78 = x ^ (y & sign),
+  // 0xD2 = x ^ (~y & sign)
+  const uint32_t t = wb << 1;  // bit 30 of wb at the sign position
+  const float sf = __uint_as_float(__builtin_amdgcn_bitop3_b32(__float_as_uint(s), t, 0x80000000u, 0xD2));
+  const float a = cp + sf;
+  const float b = cp - sf;
+  const float r1 = __uint_as_float(__builtin_amdgcn_bitop3_b32(__float_as_uint(r), wb, 0x80000000u, 0x78));
+  const float r0 = __uint_as_float(__builtin_amdgcn_bitop3_b32(__float_as_uint(r1), t, 0x80000000u, 0x78));
+  z0 = r0 * a;
+  z1 = r1 * b;
 }
 
 // Four normals of Philox call q of walk `seed`.
