@@ -151,7 +151,7 @@ int check_args(jne_ctx* ctx, int model, int dim, uint32_t steps, int precision) 
   if (model < 0 || model > 4) return set_err(ctx, JNE_ERR_INVALID_ARG, "invalid model %d (expected 0..4)", model);
   if (dim < 1 || dim > JNE_MAX_DIM)
     return set_err(ctx, JNE_ERR_INVALID_ARG, "invalid dim %d (expected 1..%d)", dim, JNE_MAX_DIM);
-  if (steps < 2 || steps > (1u << 30))
+  if (steps < 2 || steps > jne::kMaxSteps)
     return set_err(ctx, JNE_ERR_INVALID_ARG, "invalid steps %u (expected 2..2^30)", steps);
   switch (precision) {
     case JNE_PREC_FP64:

@@ -9,6 +9,12 @@ namespace jne {
 
 constexpr int kMaxDim = 12;      // the register-resident kernels (and the MFMA pass)
 constexpr int kMaxWideDim = 32;  // the runtime-dim FP64 path (jne_walk_wide.hip), dims 13..32
+constexpr uint32_t kMaxSteps = 1u << 30;  // steps per walk accepted by the C ABI
+// A walk of dim <= kMaxDim draws at most kMaxDim * kMaxSteps / 4 Philox calls,
+// so the counter's high word is zero in the register-resident kernels: they
+// count calls in 32 bits, which keeps the first rounds' products of a walk
+// whose seed is wave-uniform on the scalar unit.
+static_assert((uint64_t)kMaxDim * kMaxSteps / 4 < (1ull << 32), "32-bit Philox counters");
 
 // Per-dim shape of the FP64 moment kernel: lanes per walk (L) and target
 // waves per SIMD (the register cap).  Measured on MI355X (model 4, T = 10^4).

@@ -132,7 +132,7 @@ void walk_moments_fp64(const uint32_t* __restrict__ seeds, const double* __restr
   for (uint32_t t0 = 0; t0 < T; t0 += S) {
     // produce: this round's increments
     if (in_group) {
-      const uint64_t q0 = (uint64_t)t0 * D / 4;
+      const uint32_t q0 = (uint32_t)((uint64_t)t0 * D / 4);  // < 2^32: dim <= kMaxDim, steps <= kMaxSteps
       CallPos<D> pos = pos0;
 #if JNE_FP64_UNROLL_CALLS
 #pragma unroll
@@ -333,7 +333,7 @@ void walk_moments_fp64_reg(const uint32_t* __restrict__ seeds, const double* __r
     constexpr bool tail = decltype(tail_tag)::value;
     double w[G * D];
     if constexpr (!FROM_NORMALS) {
-      const uint64_t q0 = (uint64_t)t0 * D / 4;
+      const uint32_t q0 = (uint32_t)((uint64_t)t0 * D / 4);  // < 2^32: dim <= kMaxDim, steps <= kMaxSteps
 #pragma unroll
       for (int c = 0; c < NB; ++c) {
         float v[4];

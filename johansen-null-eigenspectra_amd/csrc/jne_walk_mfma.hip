@@ -484,7 +484,7 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
       CallPos<D> pos(c);
       for (int j = 0; j < nit; ++j) {
         float v[4];
-        normals4((uint64_t)drawn + 64 * j + l, seed, rng_stream, v);
+        normals4(drawn + 64u * j + l, seed, rng_stream, v);  // 32-bit counter (kMaxSteps)
         if (c < Sh::CALLS) {
 #pragma unroll
           for (int jj = 0; jj < 4; ++jj) {
@@ -507,7 +507,7 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
         const int c = l + 64 * j;
         const int w = c / (16 * DW), cl = c - w * (16 * DW);
         float v[4];
-        if constexpr (!FROM_NORMALS) normals4((uint64_t)ts * DW / 4 + cl, wseed[w], rng_stream, v);
+        if constexpr (!FROM_NORMALS) normals4((uint32_t)((uint64_t)ts * DW / 4) + cl, wseed[w], rng_stream, v);
         const CallPos<DW> p(cl);
 #pragma unroll
         for (int jj = 0; jj < 4; ++jj) {
@@ -531,7 +531,7 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
       const int c = l + 64 * j;
       if (Sh::IT * 64 != Sh::CALLS && c >= Sh::CALLS) break;
       float v[4];
-      if constexpr (!FROM_NORMALS) normals4((uint64_t)ts * D / 4 + c, seed, rng_stream, v);
+      if constexpr (!FROM_NORMALS) normals4((uint32_t)((uint64_t)ts * D / 4) + c, seed, rng_stream, v);
 #pragma unroll
       for (int jj = 0; jj < 4; ++jj) {
         int tl, i;
