@@ -566,9 +566,9 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
         zb[4 * q + 2] = v.z;
         zb[4 * q + 3] = v.w;
       }
-      double e = 0.0;
+      double e = (double)zb[0];  // (no 0.0 + first term: one add fewer)
 #pragma unroll
-      for (int j = 0; j < BS; ++j) e += (double)zb[j];
+      for (int j = 1; j < BS; ++j) e += (double)zb[j];
       eps_part[sp][si] = e;
     }
     __syncthreads();
@@ -578,9 +578,9 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
     // square, so the FP32 accumulation rounds less, and b_t = (B0 + s m) + s c_t.
     // Steps at or beyond T get c = 0 (their z are 0), so they add nothing.
     if (scan_lane) {
-      double e = 0.0;
+      double e = eps_part[0][si];
 #pragma unroll
-      for (int p = 0; p < SP; ++p) e += eps_part[p][si];
+      for (int p = 1; p < SP; ++p) e += eps_part[p][si];
       const float m = (float)(0.5 * e);
       // running sum in FP64 (exact but for FP64 rounding): each c_t is then
       // rounded to FP32 once
