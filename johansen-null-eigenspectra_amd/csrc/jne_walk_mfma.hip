@@ -60,6 +60,9 @@
 #ifndef JNE_MFMA_UNROLL_CALLS
 #define JNE_MFMA_UNROLL_CALLS 2
 #endif
+#ifndef JNE_MFMA_UNROLL_CALLS_FP16X2
+#define JNE_MFMA_UNROLL_CALLS_FP16X2 3
+#endif
 #ifndef JNE_MFMA_WAVES
 #define JNE_MFMA_WAVES 4
 #endif
@@ -298,13 +301,17 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float x) {
 
 // Philox calls of a segment unrolled: by-seed form only (the from-normals
 // form has no Philox chains to overlap), whole iterations only, and not with
-// the three-way bf16 split, whose operands leave no registers for it
-template <int D, int PREC, bool FROM_NORMALS>
+// the three-way bf16 split, whose operands leave no registers for it.  The
+// fp16x2 pass with one walk per wave keeps three calls in flight (all of a
+// dim-12 segment's three iterations) within its 128 VGPRs; the other operand
+// splits and the packed waves (W walks) hold more registers and keep two.
+template <int D, int PREC, bool FROM_NORMALS, int W = 1>
 struct CallUnroll {
-  static constexpr int value =
-      (JNE_MFMA_UNROLL_CALLS > 1 && !FROM_NORMALS && PREC != JNE_PREC_MFMA_BF16X3 && MfmaShape<D>::CALLS % 64 == 0)
-          ? (MfmaShape<D>::IT < JNE_MFMA_UNROLL_CALLS ? MfmaShape<D>::IT : JNE_MFMA_UNROLL_CALLS)
-          : 1;
+  static constexpr int U =
+      PREC == JNE_PREC_MFMA_FP16X2 && W == 1 ? JNE_MFMA_UNROLL_CALLS_FP16X2 : JNE_MFMA_UNROLL_CALLS;
+  static constexpr int value = (U > 1 && !FROM_NORMALS && PREC != JNE_PREC_MFMA_BF16X3 && MfmaShape<D>::CALLS % 64 == 0)
+                                   ? (MfmaShape<D>::IT < U ? MfmaShape<D>::IT : U)
+                                   : 1;
 };
 
 // Workgroups are dealt to the 8 XCDs round-robin by index, and each XCD has
@@ -502,7 +509,7 @@ void walk_moments_mfma(const uint32_t* __restrict__ seeds, const double* __restr
     } else if constexpr (W > 1) {
       // call c: local call cl of walk w, whose normals 4 cl .. 4 cl + 3 are
       // (step, component) of that walk, component i -> row w DW + i
-#pragma unroll CallUnroll<D, PREC, FROM_NORMALS>::value
+#pragma unroll CallUnroll<D, PREC, FROM_NORMALS, W>::value
       for (int j = 0; j < Sh::IT; ++j) {
         const int c = l + 64 * j;
         const int w = c / (16 * DW), cl = c - w * (16 * DW);
