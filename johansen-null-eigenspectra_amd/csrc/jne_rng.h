@@ -14,7 +14,10 @@
 // (coefficients from derive_rng_coeffs.py) and a correctly rounded square
 // root, so that every result is that of correctly rounded IEEE operations;
 // the stream is therefore bit-identical on any IEEE-754 host that performs
-// the same operation sequence.
+// the same operations.  Three steps below are written in forms that reach the
+// same values with fewer instructions (the exponent split, the logarithm at
+// twice the scale, the quadrant turn); tests/test_transform_forms.py compares
+// them with the plain forms over every input the stream can produce.
 #pragma once
 
 #include <hip/hip_runtime.h>
