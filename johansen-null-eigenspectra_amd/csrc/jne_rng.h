@@ -17,7 +17,9 @@
 // the same operations.  Three steps below are written in forms that reach the
 // same values with fewer instructions (the exponent split, the logarithm at
 // twice the scale, the quadrant turn); tests/test_transform_forms.py compares
-// them with the plain forms over every input the stream can produce.
+// them with the plain forms over every radius input the stream can produce
+// and the angle words near the quadrant edges, where a sign of zero could
+// differ.
 #pragma once
 
 #include <hip/hip_runtime.h>
