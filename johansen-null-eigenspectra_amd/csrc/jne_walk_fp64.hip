@@ -20,7 +20,7 @@
 #include "jne_internal.h"
 #include "jne_rng.h"
 
-// Tuning: unroll the steps of a round / the Philox calls of a round.
+// Tuning: unroll the steps of a round (rolled and unrolled time alike).
 #ifndef JNE_FP64_UNROLL_STEPS
 #define JNE_FP64_UNROLL_STEPS 0
 #endif
