@@ -19,7 +19,16 @@
 
 #include "jne_internal.h"
 #include "jne_rng.h"
+#include "jne_tridiag.h"
 
+// The smallest dim whose eigenvalues come from the tridiagonal QL solve
+// (jne_tridiag.h) instead of cyclic Jacobi: 1.2x (dim 7) to 1.55x (dim 12)
+// faster there; below dim 7 the two time alike (dim 5) or Jacobi leads, and
+// Jacobi keeps the higher relative accuracy of the smallest eigenvalues of
+// M = W^T W (QL's error is absolute, a few ulp of |M|, as with dggev).
+#ifndef JNE_EIGEN_QL_MIN_DIM
+#define JNE_EIGEN_QL_MIN_DIM 7
+#endif
 // Tuning: unroll the steps of a round (rolled and unrolled time alike).
 #ifndef JNE_FP64_UNROLL_STEPS
 #define JNE_FP64_UNROLL_STEPS 0
@@ -619,15 +628,28 @@ __device__ __forceinline__ void eigen_prep(const double* __restrict__ acc, size_
   pd_out = pd;
 }
 
-// Jacobi on M, |lambda| sorted descending, NX - D structural zeros; NaN rows
-// for a walk whose S11 was not positive definite.
+// sqrt and 1/x for jne_tridiag.h: the hardware estimates and one Newton step
+struct TriMath {
+  __device__ __forceinline__ static double sqrt(double x) { return x > 0.0 ? x * rsq_nr(x) : 0.0; }
+  __device__ __forceinline__ static double rcp(double x) { return rcp_nr(x); }
+};
+
+// The eigenvalues of M (tridiagonal reduction + implicit QL from dim
+// JNE_EIGEN_QL_MIN_DIM, cyclic Jacobi below it), |lambda| sorted descending,
+// NX - D structural zeros; NaN rows for a walk whose S11 was not positive
+// definite.
 template <int D, int NX>
 __device__ __forceinline__ void eigen_finish(SymReg<D>& M, bool pd, double* __restrict__ o) {
-  jacobi_eigvals<D>(M);
-
   double ev[D];
+  if constexpr (D >= JNE_EIGEN_QL_MIN_DIM) {
+    tridiag_eigvals<D, TriMath>(M.v, ev);
 #pragma unroll
-  for (int k = 0; k < D; ++k) ev[k] = fabs(M(k, k));
+    for (int k = 0; k < D; ++k) ev[k] = fabs(ev[k]);
+  } else {
+    jacobi_eigvals<D>(M);
+#pragma unroll
+    for (int k = 0; k < D; ++k) ev[k] = fabs(M(k, k));
+  }
   // sort descending (odd-even transposition, fixed network)
 #pragma unroll
   for (int pass = 0; pass < D; ++pass) {

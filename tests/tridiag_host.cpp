@@ -1,0 +1,42 @@
+// tridiag_host.cpp -- the eigenvalue solve of jne_tridiag.h (Householder +
+// implicit QL, the device's solve for dims 7-12) built for the host, for
+// test_tridiag.py.  eig(n, a, out): a is the packed upper triangle of an
+// n x n symmetric matrix (row-major, i <= j), out its n eigenvalues.
+#include <cmath>
+#include <cstring>
+
+using std::fabs;
+using std::fma;
+
+#include "jne_tridiag.h"
+
+namespace {
+struct HostMath {
+  static double sqrt(double x) { return std::sqrt(x); }
+  static double rcp(double x) { return 1.0 / x; }
+};
+template <int N>
+void run(const double* a_in, double* out) {
+  double a[N * (N + 1) / 2];
+  std::memcpy(a, a_in, sizeof a);
+  jne::tridiag_eigvals<N, HostMath>(a, out);
+}
+}  // namespace
+
+extern "C" int eig(int n, const double* a, double* out) {
+  switch (n) {
+    case 1: run<1>(a, out); return 0;
+    case 2: run<2>(a, out); return 0;
+    case 3: run<3>(a, out); return 0;
+    case 4: run<4>(a, out); return 0;
+    case 5: run<5>(a, out); return 0;
+    case 6: run<6>(a, out); return 0;
+    case 7: run<7>(a, out); return 0;
+    case 8: run<8>(a, out); return 0;
+    case 9: run<9>(a, out); return 0;
+    case 10: run<10>(a, out); return 0;
+    case 11: run<11>(a, out); return 0;
+    case 12: run<12>(a, out); return 0;
+  }
+  return 1;
+}
