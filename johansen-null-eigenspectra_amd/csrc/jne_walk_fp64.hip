@@ -704,7 +704,8 @@ __global__ __launch_bounds__(64) void walk_eigen_prep(const double* __restrict__
   m[(size_t)(D * (D + 1) / 2) * ld + w] = pd ? 1.0 : 0.0;
 }
 
-// Split form, second kernel: Jacobi from M in registers, no LDS.
+// Split form, second kernel: the eigenvalues of M in registers, no LDS
+// (eigen_finish: tridiagonal QL from dim JNE_EIGEN_QL_MIN_DIM, Jacobi below).
 template <int D, int NX>
 __global__ __launch_bounds__(64) void walk_eigen_jacobi(const double* __restrict__ m, size_t ld, uint32_t n_walks,
                                                         double* __restrict__ out) {
