@@ -1,7 +1,11 @@
-// jne_tridiag.h -- eigenvalues of a small symmetric matrix held in one
-// thread's registers: Householder reduction to tridiagonal form, then the
-// implicit QL iteration with the shift from the leading 2x2 block (the tql1
-// scheme of the EISPACK/Numerical Recipes family), eigenvalues only.
+// jne_tridiag.h -- eigenvalues of small symmetric matrices.
+//
+// tridiag_eigvals: a matrix held in one thread's registers, by Householder
+// reduction to tridiagonal form, then the implicit QL iteration with the shift
+// from the leading 2x2 block (the tql1 scheme of the EISPACK/Numerical Recipes
+// family), eigenvalues only.  sturm_*: any one eigenvalue of a tridiagonal
+// matrix by bisection on Sturm counts, so that the lanes of a wave can each
+// find one.
 //
 // Every array index is a compile-time constant once the loops are unrolled,
 // so the matrix stays in registers: the QL sweep runs over the whole trailing
@@ -137,5 +141,78 @@ JNE_TRI_HD void tridiag_eigvals(double* a, double* d) {
     }
   }
 }
+
+// ---- bisection on Sturm counts (the scheme of LAPACK's dstebz/dlaebz)
+//
+// The tridiagonal matrix T of order n <= NMAX has diagonal d[0..n-1] and
+// squared off-diagonal e2[0..n-2] (e2[i] couples i and i + 1).  The pivots of
+// T - xI = L D L^T, q_0 = d_0 - x and q_i = d_i - x - e2_{i-1} / q_{i-1}, have
+// as many negative entries as T has eigenvalues below x.  A pivot q <= pivmin
+// counts as negative and is replaced by min(q, -pivmin), which keeps every
+// quotient finite; pivmin = (smallest normal) * max(1, max e2).  Bisection
+// halves [lo, hi] (Gershgorin bounds, which hold every eigenvalue) on the
+// count, so eigenvalue k (ascending) is found independently of the others.
+
+constexpr double kSafeMin = 0x1p-1022;
+
+// Gershgorin interval of T, widened by a few ulp of its norm, and pivmin.
+template <int NMAX>
+JNE_TRI_HD void sturm_bounds(const double* d, const double* e2, int n, double& lo, double& hi, double& pivmin) {
+  double emax = 0.0, l = d[0], h = d[0];
+  double ep = 0.0;  // |e_{i-1}|
+#pragma unroll
+  for (int i = 0; i < NMAX; ++i) {
+    if (i < n) {
+      const double en = i + 1 < n ? sqrt(e2[i]) : 0.0;
+      const double r = ep + en;
+      l = fmin(l, d[i] - r);
+      h = fmax(h, d[i] + r);
+      if (i + 1 < n) emax = fmax(emax, e2[i]);
+      ep = en;
+    }
+  }
+  const double tnorm = fmax(fabs(l), fabs(h));
+  const double pad = 0x1p-50 * tnorm * (double)n;
+  pivmin = kSafeMin * fmax(1.0, emax);
+  lo = l - pad - 2.0 * pivmin;
+  hi = h + pad + 2.0 * pivmin;
+}
+
+// Number of eigenvalues of T below x.  Math::rcp(x) is 1/x to a few ulp.
+template <int NMAX, class Math>
+JNE_TRI_HD int sturm_count(const double* d, const double* e2, int n, double x, double pivmin) {
+  double q = d[0] - x;
+  bool neg = q <= pivmin;
+  int c = neg ? 1 : 0;
+  q = neg ? fmin(q, -pivmin) : q;
+#pragma unroll
+  for (int i = 1; i < NMAX; ++i) {
+    if (i < n) {
+      q = fma(-e2[i - 1], Math::rcp(q), d[i] - x);
+      neg = q <= pivmin;
+      c += neg ? 1 : 0;
+      q = neg ? fmin(q, -pivmin) : q;
+    }
+  }
+  return c;
+}
+
+// Eigenvalue k (0-based, ascending) of T by `iters` halvings of [lo, hi].
+// From the Gershgorin interval (width <= 2 |T|), 54 halvings leave a width of
+// at most 2^-53 |T|.
+template <int NMAX, class Math>
+JNE_TRI_HD double sturm_eigval(const double* d, const double* e2, int n, int k, double lo, double hi, double pivmin,
+                               int iters) {
+  for (int it = 0; it < iters; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (sturm_count<NMAX, Math>(d, e2, n, mid, pivmin) > k)
+      hi = mid;
+    else
+      lo = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
+constexpr int kBisectIters = 54;
 
 }  // namespace jne

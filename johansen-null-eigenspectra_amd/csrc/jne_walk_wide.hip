@@ -17,13 +17,16 @@
 //                      to 16, 24 or 32) and one row of the moment vectors,
 //                      FP64 fma;
 //   walk_eigen_wide    model algebra (S11 and S01 as Schur complements on the
-//                      moments), Cholesky of S11, W = L^-1 S10, M = W^T W and
-//                      cyclic Jacobi on M, with lanes working on matrix
-//                      entries, rows or columns in parallel.
+//                      moments), Cholesky of S11, W = L^-1 S10, M = W^T W,
+//                      with lanes working on matrix entries, rows or columns
+//                      in parallel; then the Householder reduction of M to
+//                      tridiagonal form (lane i holding row i in registers)
+//                      and eigenvalue k by lane k, by bisection on Sturm
+//                      counts (jne_tridiag.h).
 //
-// The arithmetic is that of walk_moments_fp64 / walk_eigen_fp64 (FP64
-// throughout, threshold Jacobi), so the results meet the same 1e-9 parity
-// with the reference's dggev solve.
+// The arithmetic is FP64 throughout, as in walk_moments_fp64 /
+// walk_eigen_fp64, so the results meet the same 1e-9 parity with the
+// reference's dggev solve.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -31,6 +34,7 @@
 
 #include "jne_internal.h"
 #include "jne_rng.h"
+#include "jne_tridiag.h"
 
 namespace jne {
 
@@ -179,17 +183,10 @@ __device__ __forceinline__ double rcp_nr_w(double x) {
   const double r = __builtin_amdgcn_rcp(x);
   return fma(r, fma(-x, r, 1.0), r);
 }
-__device__ __forceinline__ double rsq_nr_w(double x) {
-  const double r = __builtin_amdgcn_rsq(x);
-  return r * fma(-0.5 * x * r, r, 1.5);
-}
 
-// Sum over the wave's 64 lanes.
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
+struct WideMath {
+  static __device__ __forceinline__ double rcp(double x) { return rcp_nr_w(x); }
+};
 
 }  // namespace
 
@@ -212,6 +209,7 @@ size_t wide_eigen_lds_bytes(int model, int D) {
 // construct_f_matrix, src/johansen_statistics.rs:102-197), then the
 // generalized eigenproblem of src/johansen_statistics.rs:24-47 as the
 // symmetric one of M = W^T W, W = L^-1 S10, S11 = L L^T.
+template <int DP>
 __global__ __launch_bounds__(64) void walk_eigen_wide(int model, int D, const double* __restrict__ acc, size_t ld,
                                                       uint32_t T, uint32_t n_walks, double* __restrict__ out) {
   // LDS sized by the launch for this (model, dim) (wide_eigen_lds_bytes):
@@ -349,69 +347,113 @@ __global__ __launch_bounds__(64) void walk_eigen_wide(int model, int D, const do
   }
   __syncthreads();
 
-  // cyclic threshold Jacobi on M: the rotation of pair (p, q) is formed by
-  // every lane from the same three entries; lane k updates row/column k
-  for (int sweep = 0; sweep < 40; ++sweep) {
-    double off = 0.0, dia = 0.0;
-    if (l < D) {
-      dia = Ms[l][l] * Ms[l][l];
-      for (int j = l + 1; j < D; ++j) off = fma(Ms[l][j], Ms[l][j], off);
-    }
-    off = wave_sum(off);
-    dia = wave_sum(dia);
-    if (!(off > 1e-34 * dia)) break;
-    for (int p = 0; p < D - 1; ++p) {
-      for (int q = p + 1; q < D; ++q) {
-        const double apq = Ms[p][q];
-        const double app = Ms[p][p], aqq = Ms[q][q];
-        double akp = 0.0, akq = 0.0;
-        if (l < D) {
-          akp = Ms[l][p];
-          akq = Ms[l][q];
-        }
-        __syncthreads();
-        if (apq * apq <= 1e-36 * fabs(app * aqq)) {
-          // below 1e-18 sqrt(|app aqq|): zeroing a_pq moves the eigenvalues by
-          // at most ~1e-18 of themselves
-          if (l == 0) {
-            Ms[p][q] = 0.0;
-            Ms[q][p] = 0.0;
-          }
-        } else {
-          const double theta = fmin(fmax((aqq - app) * rcp_nr_w(2.0 * apq), -1e150), 1e150);
-          const double h = fma(theta, theta, 1.0);
-          double t = rcp_nr_w(fabs(theta) + h * rsq_nr_w(h));
-          if (theta < 0.0) t = -t;
-          const double c = rsq_nr_w(fma(t, t, 1.0));
-          const double s = t * c;
-          if (l < D && l != p && l != q) {
-            const double np = c * akp - s * akq;
-            const double nq = s * akp + c * akq;
-            Ms[l][p] = np;
-            Ms[p][l] = np;
-            Ms[l][q] = nq;
-            Ms[q][l] = nq;
-          }
-          if (l == 0) {
-            Ms[p][p] = app - t * apq;
-            Ms[q][q] = aqq + t * apq;
-            Ms[p][q] = 0.0;
-            Ms[q][p] = 0.0;
-          }
-        }
-        __syncthreads();
+  // ---- Householder reduction of M to tridiagonal form: lane i keeps row i
+  // of M in registers (columns padded to DP with zeros).  Step k broadcasts
+  // column k below the diagonal (x) through LDS, every lane forms the same
+  // reflector v = x - alpha e_{k+1}, tau = 2 / |v|^2 from it, lane i its entry
+  // of p = tau M v, which is broadcast in turn; then w = p - (tau/2)(p.v) v and
+  // lane i updates its row, M -= v w^T + w v^T on the trailing block.  d_k is
+  // final once step k starts and e_k = alpha.  The scratch vectors (5 DP
+  // doubles) use the bytes of W and P0, P1, at least D (D + 3) >= 5 DP doubles
+  // since DP <= D + 7.
+  double* const Xs = wide_lds + NX * (NX + 1);
+  double* const Ps = Xs + DP;
+  double* const Dg = Ps + DP;
+  double* const Eg = Dg + DP;
+  double* const EVs = Eg + DP;
+  double row[DP];
+  if (l < D) {
+#pragma unroll
+    for (int j = 0; j < DP; ++j) row[j] = j < D ? Ms[l][j < D ? j : 0] : 0.0;
+  } else {
+#pragma unroll
+    for (int j = 0; j < DP; ++j) row[j] = 0.0;
+  }
+  if (l >= D && l < DP) {  // padding entries of the broadcast vectors stay 0
+    Xs[l] = 0.0;
+    Ps[l] = 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < DP - 2; ++k) {
+    if (k < D - 2) {
+      const double xi = l > k ? row[k] : 0.0;
+      if (l > k && l < D) Xs[l] = xi;
+      __syncthreads();
+      double v[DP];
+      double sigma = 0.0;
+#pragma unroll
+      for (int j = k + 2; j < DP; ++j) {
+        v[j] = Xs[j];
+        sigma = fma(v[j], v[j], sigma);
       }
+      const double x0 = Xs[k + 1];
+      const double nrm = sqrt(fma(x0, x0, sigma));
+      const double alpha = x0 > 0.0 ? -nrm : nrm;
+      const double v0 = x0 - alpha;
+      const double vn = fma(v0, v0, sigma);
+      // tau = 0 when x = 0: the reflection is the identity, e_k = 0
+      const double tau = vn > 0.0 ? 2.0 / vn : 0.0;
+      v[k + 1] = v0;
+      const double vi = l == k + 1 ? v0 : xi;
+      double s = 0.0;
+#pragma unroll
+      for (int j = k + 1; j < DP; ++j) s = fma(row[j], v[j], s);
+      const double pi = l > k ? tau * s : 0.0;
+      if (l > k && l < D) Ps[l] = pi;
+      __syncthreads();
+      double pj[DP];
+      double pv = 0.0;
+#pragma unroll
+      for (int j = k + 1; j < DP; ++j) {
+        pj[j] = Ps[j];
+        pv = fma(pj[j], v[j], pv);
+      }
+      const double K = 0.5 * tau * pv;
+      const double wi = fma(-K, vi, pi);
+#pragma unroll
+      for (int j = k + 1; j < DP; ++j) {
+        const double wj = fma(-K, v[j], pj[j]);
+        row[j] = fma(-vi, wj, fma(-wi, v[j], row[j]));
+      }
+      if (l == 0) Eg[k] = tau > 0.0 ? alpha : x0;
+      if (l == k) Dg[k] = row[k];
     }
   }
+  // the last 2 x 2 block: d_{D-2}, d_{D-1} and e_{D-2} = M[D-1][D-2]
+  double dii = 0.0, sub = 0.0;
+#pragma unroll
+  for (int j = 0; j < DP; ++j) {
+    if (l == j) dii = row[j];
+    if (l == j + 1) sub = row[j];
+  }
+  if (l == D - 2 || l == D - 1) Dg[l] = dii;
+  if (l == D - 1) Eg[D - 2] = sub;
+  __syncthreads();
+
+  // ---- eigenvalue l (ascending) of the tridiagonal matrix by lane l, by
+  // bisection on Sturm counts from the Gershgorin interval
+  double dg[DP], e2[DP];
+#pragma unroll
+  for (int j = 0; j < DP; ++j) {
+    dg[j] = j < D ? Dg[j < D ? j : 0] : 0.0;
+    const double e = j < D - 1 ? Eg[j < D - 1 ? j : 0] : 0.0;
+    e2[j] = e * e;
+  }
+  double lo, hi, pivmin;
+  sturm_bounds<DP>(dg, e2, D, lo, hi, pivmin);
+  double lam = 0.0;
+  if (l < D) lam = sturm_eigval<DP, WideMath>(dg, e2, D, l, lo, hi, pivmin, kBisectIters);
 
   // |lambda| sorted descending (rank of each by lane), NX - D structural zeros
+  if (l < D) EVs[l] = fabs(lam);
+  __syncthreads();
   const double nan = __longlong_as_double(0x7FF8000000000000LL);
   double* o = out + walk * (size_t)NX;
   if (l < D) {
-    const double ev = fabs(Ms[l][l]);
+    const double ev = EVs[l];
     int rank = 0;
     for (int j = 0; j < D; ++j) {
-      const double x = fabs(Ms[j][j]);
+      const double x = EVs[j];
       rank += (x > ev || (x == ev && j < l)) ? 1 : 0;
     }
     o[rank] = pd ? ev : nan;
@@ -443,8 +485,13 @@ hipError_t launch_walk_moments_wide(int d, const uint32_t* seeds, const double* 
 hipError_t launch_walk_eigen_wide(int model, int d, const double* acc, size_t ld, uint32_t T, uint32_t n, double* out,
                                   hipStream_t st) {
   if (d <= kMaxDim || d > kMaxWideDim || model < 0 || model > 4) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(walk_eigen_wide, dim3(n), dim3(64), wide_eigen_lds_bytes(model, d), st, model, d, acc, ld, T, n,
-                     out);
+  const size_t lds = wide_eigen_lds_bytes(model, d);
+  if (d <= 16)
+    hipLaunchKernelGGL(walk_eigen_wide<16>, dim3(n), dim3(64), lds, st, model, d, acc, ld, T, n, out);
+  else if (d <= 24)
+    hipLaunchKernelGGL(walk_eigen_wide<24>, dim3(n), dim3(64), lds, st, model, d, acc, ld, T, n, out);
+  else
+    hipLaunchKernelGGL(walk_eigen_wide<32>, dim3(n), dim3(64), lds, st, model, d, acc, ld, T, n, out);
   return hipGetLastError();
 }
 

@@ -43,6 +43,21 @@ def test_wide_dims_long_walks(engine, O, model, dim, steps):
     assert_eigs_close(got, ref, RTOL, f"model {model} dim {dim} steps {steps}")
 
 
+def test_wide_tiny_smallest_eigenvalue(engine, O):
+    """Walks whose smallest eigenvalue is 4e-11 .. 2e-9 of the largest (model 4,
+    T = 10^4): the Householder + bisection solve keeps to the bar there, where
+    the 1e-13 rho_max term dominates (dggev itself is off by ~1e-6 relative on
+    such an eigenvalue against a 40-digit solve of the same moments)."""
+    for dim, seeds in ((32, [2710, 1180]), (20, [2230, 2274])):
+        seeds = np.array(seeds, dtype=np.uint32)
+        got = engine.eigenvalues_by_seed(4, dim, 10000, seeds)
+        ref = O.walks_by_seed_c(4, dim, 10000, seeds, threads=2)
+        assert (np.abs(ref).min(axis=1) < 3e-9 * np.abs(ref).max(axis=1)).all()
+        assert_eigs_close(got, ref, RTOL, f"model 4 dim {dim} steps 10000 (C oracle)")
+        lap = O.walk_eigenvalues_by_seed(4, dim, 10000, int(seeds[0]))
+        assert_eigs_close(got[0], lap, RTOL, f"model 4 dim {dim} steps 10000 (dggev oracle)")
+
+
 def test_wide_arbitrary_increments_and_from_normals(engine, O):
     rng = np.random.default_rng(13)
     for model in range(5):

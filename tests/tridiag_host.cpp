@@ -1,12 +1,20 @@
-// tridiag_host.cpp -- the eigenvalue solve of jne_tridiag.h (Householder +
-// implicit QL, the device's solve for dims 7-12) built for the host, for
-// test_tridiag.py.  eig(n, a, out): a is the packed upper triangle of an
-// n x n symmetric matrix (row-major, i <= j), out its n eigenvalues.
+// tridiag_host.cpp -- the eigenvalue solves of jne_tridiag.h built for the
+// host, for test_tridiag.py.
+//   eig(n, a, out): Householder + implicit QL (the device's solve for dims
+//     7-12); a is the packed upper triangle of an n x n symmetric matrix
+//     (row-major, i <= j), out its n eigenvalues.
+//   sturm_eigs(n, d, e, out): bisection on Sturm counts (the device's solve of
+//     the tridiagonal form for dims 13-32, one eigenvalue per lane); d, e the
+//     diagonal and off-diagonal of an n x n tridiagonal matrix, n <= 32, out
+//     its eigenvalues in ascending order.
 #include <cmath>
 #include <cstring>
 
 using std::fabs;
 using std::fma;
+using std::fmax;
+using std::fmin;
+using std::sqrt;
 
 #include "jne_tridiag.h"
 
@@ -39,4 +47,17 @@ extern "C" int eig(int n, const double* a, double* out) {
     case 12: run<12>(a, out); return 0;
   }
   return 1;
+}
+
+extern "C" int sturm_eigs(int n, const double* d, const double* e, double* out) {
+  constexpr int NMAX = 32;
+  if (n < 1 || n > NMAX) return 1;
+  double dd[NMAX] = {}, e2[NMAX] = {};
+  for (int i = 0; i < n; ++i) dd[i] = d[i];
+  for (int i = 0; i + 1 < n; ++i) e2[i] = e[i] * e[i];
+  double lo, hi, pivmin;
+  jne::sturm_bounds<NMAX>(dd, e2, n, lo, hi, pivmin);
+  for (int k = 0; k < n; ++k)
+    out[k] = jne::sturm_eigval<NMAX, HostMath>(dd, e2, n, k, lo, hi, pivmin, jne::kBisectIters);
+  return 0;
 }
