@@ -25,11 +25,13 @@ def server(jne):
 
 
 def test_query_matches_batch_path(jne, engine, O, server):
-    # the last two span several chunks: 65,536 walks at most, and at dim 12,
-    # T = 10,000 at most 2^31 normals (17,664 walks), the last one partial
+    # the third and fourth span several chunks: 65,536 walks at most, and at
+    # dim 12, T = 10,000 at most 2^31 normals (17,664 walks), the last one
+    # partial; the last two run the wide-dim kernels (one wave per walk)
     for model, dim, steps, runs, seed, prec in [(2, 3, 500, 20000, 7, "fp64"), (4, 12, 1000, 3000, 11, "mfma_fp16x2"),
                                                (1, 1, 100, 70001, 3, "mfma_fp16x2"),
-                                               (4, 12, 10000, 40000, 5, "mfma_fp16x2")]:
+                                               (4, 12, 10000, 40000, 5, "mfma_fp16x2"),
+                                               (0, 13, 300, 20000, 21, "auto"), (4, 32, 500, 9000, 23, "fp64")]:
         q, lat = server.wait(server.submit(model, dim, steps, runs, seed, prec))
         assert np.array_equal(q, expected(engine, O, model, dim, steps, runs, seed, jne.PRECISIONS[prec]))
         assert lat > 0
