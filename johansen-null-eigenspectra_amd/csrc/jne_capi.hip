@@ -292,12 +292,16 @@ hipStream_t stream_of(jne_ctx* ctx, void* hip_stream) {
   return hip_stream == JNE_STREAM_CONTEXT ? ctx->stream : (hipStream_t)hip_stream;
 }
 
-int finish_host_out(jne_ctx* ctx, double* out, size_t count) {
+// n rows of nf eigenvalues from the context's device buffer; a NaN row (S11
+// not positive definite) makes the call return JNE_ERR_NUMERIC, naming the
+// first such walk (0-based row).
+int finish_host_out(jne_ctx* ctx, double* out, size_t n, size_t nf) {
+  const size_t count = n * nf;
   JNE_HIP_CHECK(ctx, hipMemcpyAsync(out, ctx->d_out, count * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   JNE_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   for (size_t i = 0; i < count; ++i)
     if (isnan(out[i]))
-      return set_err(ctx, JNE_ERR_NUMERIC, "walk %zu: S11 is not positive definite", i);
+      return set_err(ctx, JNE_ERR_NUMERIC, "walk %zu: S11 is not positive definite", i / nf);
   return JNE_OK;
 }
 
@@ -393,7 +397,7 @@ int jne_eigenvalues_by_seed(jne_ctx* ctx, int model, int dim, uint32_t steps, ui
   if ((rc = run_walks(ctx, &model, 1, dim, steps, rng_stream, (const uint32_t*)ctx->d_in, nullptr, n, precision, outs,
                       ctx->stream)))
     return rc;
-  return finish_host_out(ctx, out, n * nf);
+  return finish_host_out(ctx, out, n, nf);
 }
 
 int jne_eigenvalues_by_seed_device(jne_ctx* ctx, int model, int dim, uint32_t steps, uint32_t rng_stream,
@@ -477,7 +481,7 @@ int jne_eigenvalues_from_normals(jne_ctx* ctx, int model, int dim, uint32_t step
   if ((rc = run_walks(ctx, &model, 1, dim, steps, 0, nullptr, (const double*)ctx->d_in, n, precision, outs,
                       ctx->stream)))
     return rc;
-  return finish_host_out(ctx, out, n * nf);
+  return finish_host_out(ctx, out, n, nf);
 }
 
 namespace {

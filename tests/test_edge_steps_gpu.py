@@ -8,7 +8,8 @@ blocks (29 = 2 x 12 + 5, 37 = 3 x 12 + 1, 63 = 0 x 64 + 63, 65 = 64 + 1,
 FP64 is held to the oracle (C restatement of the reference walk) at
 |d| <= 1e-9 |rho| + 1e-13 rho_max; the reduced precisions to the FP64 path
 at the short-walk normwise bound, and the from-normals entry point to the
-by-seed one bit for bit.
+by-seed one bit for bit.  A walk of zero increments (S11 = 0, not positive
+definite) gives a NaN row on each form of the eigen solve.
 """
 import numpy as np
 import pytest
@@ -55,3 +56,25 @@ def test_mixed_segment_remainders(engine, jne, prec, dim):
             assert err <= TOL_SHORT[prec], f"{prec} model {model} dim {dim} steps {steps}: {err:.2e}"
         z = engine.walk_normals(dim, steps, seeds).astype(np.float64)
         assert np.array_equal(engine.eigenvalues_from_normals(4, z, precision=p), got), (prec, dim, steps)
+
+
+@pytest.mark.parametrize("dim", [3, 12, 20])
+def test_degenerate_walk_gives_nan_row(jne, engine, O, dim):
+    """A walk whose increments are all zero has S11 = 0 (not positive
+    definite): the call returns JNE_ERR_NUMERIC naming that walk and its row
+    is NaN, on the fused (dim 3), split (dim 12) and one-wave-per-walk (dim 20)
+    solves alike, and the walks beside it in the batch keep their values."""
+    import ctypes
+    rng = np.random.default_rng(dim)
+    for model in range(5):
+        z = np.ascontiguousarray(rng.standard_normal((3, 64, dim)))
+        z[1] = 0.0
+        out = np.full((3, jne.num_eigenvalues(model, dim)), 7.0)
+        rc = engine._lib.jne_eigenvalues_from_normals(engine._ctx, model, dim, 64, z.ctypes.data, 3,
+                                                      jne.PRECISIONS["fp64"], out.ctypes.data)
+        assert rc == jne.JNE_ERR_NUMERIC
+        assert engine._lib.jne_last_error(engine._ctx).decode().startswith("walk 1:")
+        assert np.isnan(out[1]).all()
+        assert np.isfinite(out[[0, 2]]).all()
+        ref = np.stack([O.walk_eigenvalues_lapack(model, z[k]) for k in (0, 2)])
+        assert_eigs_close(out[[0, 2]], ref, 1e-9, f"model {model} dim {dim}")
