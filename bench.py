@@ -499,6 +499,25 @@ def config1_full(pkg, eng, threads, host, model=0, dim=2, T=1000, runs=100_000):
             "gpu_vs_cpu_max_diff_over_fp64_bar": float(bar.max())}
 
 
+def wide_dims(pkg, eng, torch, stream, dims=(20, 32), model=4, T=10_000, n=65_536, steps=3):
+    """The one-wave-per-walk FP64 path above dim 12 (the reference's operator
+    takes any dim, src/johansen_statistics.rs:59-85): walks/s of the whole path
+    and the time per launch of its moment and eigen kernels, at model 4,
+    T = 10^4, n walks per launch."""
+    dev = torch.device("cuda:0")
+    seeds = torch.arange(1, n + 1, dtype=torch.int32, device=dev)
+    rows = []
+    for dim in dims:
+        out = torch.empty(n * pkg.num_eigenvalues(model, dim), dtype=torch.float64, device=dev)
+        wps, _sps, mom, eig_ms = time_path(pkg, eng, "fp64", model, dim, T, seeds, out, n, steps, 1, stream, torch)
+        assert bool(torch.isfinite(out).all()), "non-finite eigenvalues"
+        launches = max(mom[1], 1)
+        rows.append({"dim": dim, "walks_per_s": wps, "moments_ms_per_launch": mom[0] / launches,
+                     "eigen_ms_per_launch": eig_ms / launches})
+    return {"workload": f"Johansen model {model}, dims {', '.join(map(str, dims))}, T={T:,} steps, {n:,} walks per "
+                        f"launch, FP64 path with one wave per walk", "dims": rows}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -517,6 +536,7 @@ def main() -> int:
                     help="0 = every CPU of this job's share of the host (affinity, cgroup quota, OMP_NUM_THREADS)")
     ap.add_argument("--no-config3", action="store_true",
                     help="skip the production sweep sample (models 0-4 x dims 1-12, T=10^4, 10^5 runs per pair)")
+    ap.add_argument("--no-wide", action="store_true", help="skip the dims-20 and 32 timing of the wide-dim path")
     ap.add_argument("--no-config1", action="store_true",
                     help="skip the plain-CPU configuration (model 0, dim 2, T=1,000, 100,000 runs) on CPU and GPU")
     ap.add_argument("--serve-rates", default="10,30,100,300,1000",
@@ -644,6 +664,8 @@ def main() -> int:
             sv["saturated_mixed_vs_fp64"] = sv["rates"][-1]["walk_steps_per_s"] / top["walk_steps_per_s"]
         line["serving"] = sv
 
+    if not args.no_wide:
+        line["wide_dims"] = wide_dims(pkg, eng, torch, stream)
     host = host_cpu_info()
     threads = args.cpu_threads or host["cpu_share"]
     if not args.no_config1:

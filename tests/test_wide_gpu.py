@@ -22,7 +22,7 @@ CLI = os.path.join(ROOT, "johansen-null-eigenspectra_amd", "bin", "johansen-null
 
 
 @pytest.mark.parametrize("model", [0, 1, 2, 3, 4])
-@pytest.mark.parametrize("dim", [13, 16, 20, 25, 32])
+@pytest.mark.parametrize("dim", [13, 16, 17, 20, 24, 25, 32])  # both ends of each padded class
 def test_wide_dims_match_oracle(engine, O, model, dim):
     steps = 301 if dim % 2 else 256  # a partial last round
     seeds = np.array([1, 2, 0xFFFFFFFF], dtype=np.uint32)
