@@ -128,20 +128,17 @@ hipError_t launch_walk_normals(int d, const uint32_t* seeds, uint32_t rng_stream
 constexpr int kMaxPercentiles = 8;
 constexpr int kMaxArrays = 2;  // arrays of one call (the server's trace and max-eig)
 struct QuantileScratch {
-  void* small = nullptr;  // histogram, selection state, gathered values
+  void* small = nullptr;  // histogram, selection state, gathered values, radix-select state
   size_t small_bytes = 0;
-  double* sorted = nullptr;  // the fallback's sorted copy
-  size_t sorted_bytes = 0;
-  void* temp = nullptr;  // the fallback's radix-sort storage
-  size_t temp_bytes = 0;
   ~QuantileScratch();
 };
 // out[a*np + i] = percentile p[i] (0 <= p[i] <= 1, np <= kMaxPercentiles) of
 // xs[a][0..n), a < nx <= kMaxArrays, 1 <= n < 2^31, by the rule of
-// get_percentile_value; one read-back, synchronous on st.  Returns 0,
-// JNE_ERR_NUMERIC (a NaN in an array) or another JNE_ERR_* code; *used_sort
-// tells whether an array was sorted instead of selected from.
+// get_percentile_value; synchronous on st.  Returns 0, JNE_ERR_NUMERIC (a NaN
+// in an array) or another JNE_ERR_* code; *used_fallback tells whether an
+// array's order statistics came from the radix select instead of the bucket
+// selection.
 int quantile_select(QuantileScratch& sc, const double* const* xs, int nx, uint64_t n, const double* p, int np,
-                    double* out, hipStream_t st, bool* used_sort);
+                    double* out, hipStream_t st, bool* used_fallback);
 
 }  // namespace jne

@@ -18,10 +18,10 @@
 //
 // When the gathered buckets hold more than kMaxCand values (heavy ties, or a
 // few extreme values stretching the bucket width), or min/max are not finite,
-// the array is sorted instead (hipCUB radix sort into scratch).  Either way
+// the order statistics come from a radix select instead: eight passes of
+// 8-bit digits over the order-preserving 64-bit keys, MSB first.  Either way
 // the results are the exact order statistics, identical to sorting.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 #include <math.h>
 #include <stdint.h>
 
@@ -49,7 +49,7 @@ __device__ __forceinline__ uint64_t ord_key(double x) {
 // Bucket selection state of one array, written by qs_select and read by
 // qs_gather/qs_finish.
 struct Sel {
-  uint32_t status;  // kSelect, kSort, kNan or kEqual
+  uint32_t status;  // kSelect, kRadix, kNan or kEqual
   uint32_t nb;      // distinct buckets holding needed order statistics
   uint32_t m;       // values gathered
   uint32_t bucket[kMaxRanks];
@@ -58,7 +58,7 @@ struct Sel {
   uint32_t fill[kMaxRanks];     // gather cursors
   uint32_t pos[kMaxRanks];      // slot of rank k among the sorted gathered values
 };
-constexpr uint32_t kSelect = 0, kSort = 1, kNan = 2, kEqual = 3;
+constexpr uint32_t kSelect = 0, kRadix = 1, kNan = 2, kEqual = 3;
 
 // Everything a launch needs, passed by value: no host-to-device copies.
 struct QsArgs {
@@ -139,7 +139,7 @@ __device__ __forceinline__ uint32_t grid_of(const QsArray& a, double& lo, double
   if (a.mm[2]) return kNan;
   if (lo == hi) return kEqual;
   const double width = hi - lo;
-  if (!(isfinite(lo) && isfinite(hi) && isfinite(width) && width > 0.0)) return kSort;
+  if (!(isfinite(lo) && isfinite(hi) && isfinite(width) && width > 0.0)) return kRadix;
   scale = (double)kBuckets / width;
   return kSelect;
 }
@@ -236,7 +236,7 @@ __global__ __launch_bounds__(256) void qs_select(QsArgs args, QsArray* arr) {
         if (sel->bucket[j] == rb[k]) sel->pos[k] = sel->base[j] + (uint32_t)rl[k];
     sel->nb = nb;
     sel->m = m;
-    sel->status = over ? kSort : kSelect;
+    sel->status = over ? kRadix : kSelect;
   }
 }
 
@@ -309,12 +309,82 @@ __global__ __launch_bounds__(1024) void qs_finish(QsArgs args, const QsArray* ar
                       v[sel.pos[2 * t + 1]]);
 }
 
-// Percentiles from a fully sorted array (the fallback).
-__global__ void qs_from_sorted(QsArgs args, const double* __restrict__ s, QsResult* res) {
+// ---- the fallback: MSB-first radix select on the keys of ord_key.  Pass L
+// histograms digit L (8 bits) of the keys whose first L digits equal the
+// prefix found so far for each needed rank, one histogram per rank; the pick
+// step extends each prefix by the digit whose cumulative count passes the
+// rank, and the rank becomes the rank among the keys of that digit.  After
+// the eighth pass each prefix is the whole key of its order statistic, exact
+// whatever the ties, outliers or infinities, with no sort of the array.
+constexpr int kDigits = 8;
+
+struct Radix {
+  uint64_t prefix[kMaxRanks];  // the digits found so far
+  uint64_t k[kMaxRanks];       // rank among the keys with that prefix
+  uint32_t hist[kMaxRanks][256];
+};
+
+__global__ void qs_radix_init(QsArgs args, Radix* rx) {
+  Radix& r = rx[blockIdx.y];
+  const int t = threadIdx.x;
+  if (t < 2 * args.np) {
+    r.prefix[t] = 0;
+    r.k[t] = args.ranks[t];
+  }
+  for (int e = t; e < kMaxRanks * 256; e += blockDim.x) (&r.hist[0][0])[e] = 0u;
+}
+
+__global__ __launch_bounds__(256) void qs_radix_hist(QsArgs args, Radix* rx, int level) {
+  __shared__ uint32_t h[kMaxRanks][256];
+  __shared__ uint64_t pre[kMaxRanks];
+  const double* __restrict__ x = args.x[blockIdx.y];
+  Radix& r = rx[blockIdx.y];
+  const int t = threadIdx.x;
+  const int nr = 2 * args.np;
+  for (int e = t; e < nr * 256; e += 256) (&h[0][0])[e] = 0u;
+  if (t < nr) pre[t] = r.prefix[t];
+  __syncthreads();
+  const int shift = 56 - 8 * level;  // this pass's digit
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + t; i < args.n; i += (uint64_t)gridDim.x * 256) {
+    const uint64_t key = ord_key(x[i]);
+    const uint32_t d = (uint32_t)(key >> shift) & 255u;
+    const uint64_t high = level == 0 ? 0ull : key >> (shift + 8);
+    for (int k = 0; k < nr; ++k)
+      if (high == pre[k]) atomicAdd(&h[k][d], 1u);
+  }
+  __syncthreads();
+  for (int e = t; e < nr * 256; e += 256) {
+    const uint32_t c = (&h[0][0])[e];
+    if (c) atomicAdd(&(&r.hist[0][0])[e], c);
+  }
+}
+
+// One thread per needed rank; the histograms are then cleared for the next pass.
+__global__ void qs_radix_pick(QsArgs args, Radix* rx) {
+  Radix& r = rx[blockIdx.y];
+  const int t = threadIdx.x;
+  if (t < 2 * args.np) {
+    const uint64_t k = r.k[t];
+    uint64_t cum = 0;
+    uint32_t d = 0;
+    for (; d < 255u; ++d) {  // the keys with this prefix number more than k
+      const uint64_t c = r.hist[t][d];
+      if (k < cum + c) break;
+      cum += c;
+    }
+    r.k[t] = k - cum;
+    r.prefix[t] = (r.prefix[t] << 8) | d;
+  }
+  __syncthreads();
+  for (int e = t; e < kMaxRanks * 256; e += blockDim.x) (&r.hist[0][0])[e] = 0u;
+}
+
+__global__ void qs_radix_finish(QsArgs args, const Radix* rx, QsResult* res) {
+  const Radix& r = rx[blockIdx.y];
   const int t = threadIdx.x;
   if (t < args.np)
-    res->out[t] = interp(args.prank[t], args.ranks[2 * t], args.ranks[2 * t + 1], s[args.ranks[2 * t]],
-                         s[args.ranks[2 * t + 1]]);
+    res[blockIdx.y].out[t] = interp(args.prank[t], args.ranks[2 * t], args.ranks[2 * t + 1], from_key(r.prefix[2 * t]),
+                                    from_key(r.prefix[2 * t + 1]));
 }
 
 hipError_t grow(void** p, size_t* have, size_t need) {
@@ -331,20 +401,20 @@ hipError_t grow(void** p, size_t* have, size_t need) {
 
 QuantileScratch::~QuantileScratch() {
   if (small) (void)hipFree(small);
-  if (sorted) (void)hipFree(sorted);
-  if (temp) (void)hipFree(temp);
 }
 
 int quantile_select(QuantileScratch& sc, const double* const* xs, int nx, uint64_t n, const double* p, int np,
-                    double* out, hipStream_t st, bool* used_sort) {
-  if (used_sort) *used_sort = false;
+                    double* out, hipStream_t st, bool* used_fallback) {
+  if (used_fallback) *used_fallback = false;
   if (nx < 1 || nx > kMaxArrays || np < 1 || np > kMaxPercentiles || n < 1 || n > 0x7FFFFFFFull)
     return JNE_ERR_INVALID_ARG;
   const size_t res_off = ((size_t)kMaxArrays * sizeof(QsArray) + 255) & ~size_t(255);
-  hipError_t e = grow(&sc.small, &sc.small_bytes, res_off + kMaxArrays * sizeof(QsResult));
+  const size_t radix_off = (res_off + kMaxArrays * sizeof(QsResult) + 255) & ~size_t(255);
+  hipError_t e = grow(&sc.small, &sc.small_bytes, radix_off + kMaxArrays * sizeof(Radix));
   if (e != hipSuccess) return JNE_ERR_HIP;
   QsArray* arr = static_cast<QsArray*>(sc.small);
   QsResult* d_res = reinterpret_cast<QsResult*>(static_cast<char*>(sc.small) + res_off);
+  Radix* radix = reinterpret_cast<Radix*>(static_cast<char*>(sc.small) + radix_off);
 
   // ranks of the needed order statistics (get_percentile_value's rank)
   QsArgs args = {};
@@ -371,20 +441,21 @@ int quantile_select(QuantileScratch& sc, const double* const* xs, int nx, uint64
   for (int a = 0; a < nx; ++a)
     if (res[a].status == kNan) return JNE_ERR_NUMERIC;
   for (int a = 0; a < nx; ++a) {
-    if (res[a].status != kSort) {
+    if (res[a].status != kRadix) {
       memcpy(out + (size_t)a * np, res[a].out, np * sizeof(double));
       continue;
     }
-    // fallback: radix sort into scratch, then the order statistics
-    if (used_sort) *used_sort = true;
-    if ((e = grow(reinterpret_cast<void**>(&sc.sorted), &sc.sorted_bytes, n * sizeof(double))) != hipSuccess)
-      return JNE_ERR_HIP;
-    size_t need = 0;
-    if ((e = hipcub::DeviceRadixSort::SortKeys(nullptr, need, xs[a], sc.sorted, (int)n, 0, 64, st)) != hipSuccess ||
-        (e = grow(&sc.temp, &sc.temp_bytes, need)) != hipSuccess ||
-        (e = hipcub::DeviceRadixSort::SortKeys(sc.temp, need, xs[a], sc.sorted, (int)n, 0, 64, st)) != hipSuccess)
-      return JNE_ERR_HIP;
-    hipLaunchKernelGGL(qs_from_sorted, dim3(1), dim3(64), 0, st, args, sc.sorted, d_res + a);
+    // the fallback: radix select of this array's order statistics
+    if (used_fallback) *used_fallback = true;
+    QsArgs one = args;
+    one.x[0] = xs[a];
+    Radix* const rx = radix + a;
+    hipLaunchKernelGGL(qs_radix_init, dim3(1), dim3(256), 0, st, one, rx);
+    for (int level = 0; level < kDigits; ++level) {
+      hipLaunchKernelGGL(qs_radix_hist, dim3(std::min(grid, 512u)), dim3(256), 0, st, one, rx, level);
+      hipLaunchKernelGGL(qs_radix_pick, dim3(1), dim3(256), 0, st, one, rx);
+    }
+    hipLaunchKernelGGL(qs_radix_finish, dim3(1), dim3(64), 0, st, one, rx, d_res + a);
     if ((e = hipGetLastError()) != hipSuccess ||
         (e = hipMemcpyAsync(out + (size_t)a * np, d_res[a].out, np * sizeof(double), hipMemcpyDeviceToHost, st)) !=
             hipSuccess ||

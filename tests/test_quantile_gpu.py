@@ -6,8 +6,9 @@ Exact order statistics and the same interpolation arithmetic, so the values
 must be equal exactly (bit for bit but for the sign of a zero: -0.0 and 0.0
 are equal values and their order in a sort is not defined).  Cases: sizes from 1 to 2^21, smooth and heavy-tailed
 data, negative values and signed zeros, ties (every value equal; three
-distinct values, whose buckets overflow the selection and go to the sort),
-a few extreme outliers (a stretched bucket width), infinities (the sort),
+distinct values, whose buckets overflow the selection and go to the radix
+select), a few extreme outliers (a stretched bucket width), infinities (the
+radix select),
 NaN (an error) and invalid arguments.
 """
 import numpy as np
@@ -52,17 +53,17 @@ def test_distributions(engine, O, dist):
 
 def test_ties_and_signed_zeros(engine, O):
     check(engine, O, np.full(50_000, 3.25))                       # all equal
-    check(engine, O, np.repeat([1.0, 2.0, 5.0], [40_000, 5_000, 5_000]))  # crowded buckets: the sort
+    check(engine, O, np.repeat([1.0, 2.0, 5.0], [40_000, 5_000, 5_000]))  # crowded buckets: the radix select
     check(engine, O, np.array([0.0, -0.0, 0.0, -0.0, 1.0]), [0.0, 0.25, 0.5, 1.0])
 
 
 def test_outliers_and_infinities(engine, O):
     rng = np.random.default_rng(3)
     x = rng.standard_normal(300_000)
-    x[:3] = [1e300, -1e300, 5e299]          # a stretched bucket width: the sort
+    x[:3] = [1e300, -1e300, 5e299]          # a stretched bucket width: the radix select
     check(engine, O, x)
     y = rng.standard_normal(100_000)
-    y[10] = np.inf                         # not finite: the sort
+    y[10] = np.inf                         # not finite: the radix select
     check(engine, O, y)
     y[11] = -np.inf
     check(engine, O, y, [0.0, 0.5, 1.0])
@@ -80,3 +81,23 @@ def test_errors(engine, jne):
         engine.percentiles_device(ok.ptr, 1 << 31, PS)
     assert all(np.isnan(engine.percentiles_device(ok.ptr, 0, PS)))   # as jne_percentiles
     assert engine.percentiles_device(ok.ptr, 10, [0.5]) == [4.5]
+
+
+def test_radix_select_fallback(engine, O):
+    """Arrays the bucket selection hands to the radix select (an infinity, or
+    a stretched bucket width), built so that the order statistics differ only
+    in their last key digits, sit in long runs of ties, or lie among negative
+    values and signed zeros."""
+    rng = np.random.default_rng(11)
+    ulps = 1.0 + np.arange(60_000) * np.finfo(np.float64).eps  # keys equal but for the low digits
+    rng.shuffle(ulps)
+    check(engine, O, np.concatenate([ulps, [np.inf]]), [0.0, 0.1, 0.5, 0.9, 0.95, 0.99])
+    ties = np.repeat(rng.standard_normal(40), 2_500)  # 40 values, 2,500 copies each
+    check(engine, O, np.concatenate([ties, [1e308, -1e308]]), [0.0, 0.05, 0.5, 0.9, 0.95, 0.99, 1.0])
+    neg = -np.abs(rng.standard_normal(200_000))
+    neg[:1000] = -0.0
+    neg[1000:2000] = 0.0
+    check(engine, O, np.concatenate([neg, [-np.inf]]), [0.0, 0.5, 0.9, 0.995, 1.0])
+    big = rng.standard_normal(1 << 21)
+    big[123] = np.inf
+    check(engine, O, big)
