@@ -36,7 +36,8 @@ JNE_TRI_HD constexpr int tri_idx(int i, int j) {
   return i <= j ? i * N - i * (i - 1) / 2 + (j - i) : j * N - j * (j - 1) / 2 + (i - j);
 }
 
-// Math supplies sqrt(x) for x >= 0 and 1/x (x != 0), each to a few ulp.
+// Math supplies sqrt(x) for x >= 0, 1/x (x != 0) and 1/sqrt(x) (x > 0), each
+// to a few ulp.
 template <int N, class Math>
 JNE_TRI_HD void tridiag_eigvals(double* a, double* d) {
   double e[N];  // e[i] couples d[i] and d[i + 1]; e[N - 1] = 0
@@ -115,8 +116,8 @@ JNE_TRI_HD void tridiag_eigvals(double* a, double* d) {
         const bool act = i < m && !split;
         const double f = s * e[i], b = c * e[i];
         const double h = fma(f, f, g * g);
-        const double rr = Math::sqrt(h);
-        const double ri = rr > 0.0 ? Math::rcp(rr) : 0.0;
+        const double ri = h > 0.0 ? Math::rsqrt(h) : 0.0;  // 1 / |(f, g)|
+        const double rr = h * ri;                           // |(f, g)|
         const double sn = f * ri, cn = g * ri;
         const double g2 = d[i + 1] - p;
         const double r2 = fma(d[i] - g2, sn, 2.0 * cn * b);

@@ -628,10 +628,12 @@ __device__ __forceinline__ void eigen_prep(const double* __restrict__ acc, size_
   pd_out = pd;
 }
 
-// sqrt and 1/x for jne_tridiag.h: the hardware estimates and one Newton step
+// sqrt, 1/x and 1/sqrt(x) for jne_tridiag.h: the hardware estimates and one
+// Newton step
 struct TriMath {
   __device__ __forceinline__ static double sqrt(double x) { return x > 0.0 ? x * rsq_nr(x) : 0.0; }
   __device__ __forceinline__ static double rcp(double x) { return rcp_nr(x); }
+  __device__ __forceinline__ static double rsqrt(double x) { return rsq_nr(x); }
 };
 
 // The eigenvalues of M (tridiagonal reduction + implicit QL from dim

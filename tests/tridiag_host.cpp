@@ -22,6 +22,7 @@ namespace {
 struct HostMath {
   static double sqrt(double x) { return std::sqrt(x); }
   static double rcp(double x) { return 1.0 / x; }
+  static double rsqrt(double x) { return 1.0 / std::sqrt(x); }
 };
 template <int N>
 void run(const double* a_in, double* out) {
