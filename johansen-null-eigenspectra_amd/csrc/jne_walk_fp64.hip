@@ -565,8 +565,10 @@ __device__ __forceinline__ void eigen_prep(const double* __restrict__ acc, size_
     }
   };
 
-  // S11 (lower triangle in SymReg form) and its Cholesky factor in place
+  // S11 (lower triangle in SymReg form) and its Cholesky factor in place;
+  // dinv[j] = 1 / L(j, j), which the substitutions below multiply by
   SymReg<NX> Lf;
+  double dinv[NX];
   bool pd = true;
 #pragma unroll
   for (int i = 0; i < NX; ++i) {
@@ -582,6 +584,7 @@ __device__ __forceinline__ void eigen_prep(const double* __restrict__ acc, size_
     const double ljj = sqrt(s > 0.0 ? s : 1.0);
     const double inv = 1.0 / ljj;
     Lf(j, j) = ljj;
+    dinv[j] = inv;
 #pragma unroll
     for (int i = j + 1; i < NX; ++i) {
       double v = Lf(i, j);
@@ -606,7 +609,7 @@ __device__ __forceinline__ void eigen_prep(const double* __restrict__ acc, size_
       }
 #pragma unroll
       for (int j = 0; j < i; ++j) v -= Lf(i, j) * wv[j];
-      wv[i] = v / Lf(i, i);
+      wv[i] = v * dinv[i];
       wsh[i * D + k][tid] = wv[i];
     }
   }
