@@ -307,7 +307,10 @@ __global__ __launch_bounds__(64) void walk_eigen_wide(int model, int D, const do
   __syncthreads();
 
   // Cholesky S11 = L L^T, one column at a time: every lane forms the pivot
-  // (the same value), lanes below it their entry of the column
+  // (the same value), lanes below it their entry of the column.  The
+  // reciprocal pivots go to P0's bytes (P0 and P1 are not read again) for the
+  // substitution below.
+  double* const dinv = P0;
   bool pd = true;
   for (int j = 0; j < NX; ++j) {
     double s = Ls[j][j];
@@ -322,7 +325,10 @@ __global__ __launch_bounds__(64) void walk_eigen_wide(int model, int D, const do
       for (int k = 0; k < j; ++k) v -= Ls[i][k] * Ls[j][k];
     }
     __syncthreads();
-    if (l == 0) Ls[j][j] = ljj;
+    if (l == 0) {
+      Ls[j][j] = ljj;
+      dinv[j] = inv;
+    }
     if (i < NX) Ls[i][j] = v * inv;
     __syncthreads();
   }
@@ -332,7 +338,7 @@ __global__ __launch_bounds__(64) void walk_eigen_wide(int model, int D, const do
     for (int i = 0; i < NX; ++i) {
       double v = Ws[i][l];
       for (int j = 0; j < i; ++j) v -= Ls[i][j] * Ws[j][l];
-      Ws[i][l] = v / Ls[i][i];
+      Ws[i][l] = v * dinv[i];
     }
   }
   __syncthreads();
