@@ -42,6 +42,7 @@ namespace {
 
 constexpr int kPadDim = kMaxWideDim;            // components padded to 32
 constexpr int kRoundMax = 16;                   // most steps in one round (dim 13..16)
+static_assert(kPadDim + kRoundMax <= 64, "lanes past the components form the steps' u and u^2");
 
 // Steps per round: a multiple of 4 (whole Philox calls for any dim) with at
 // most 64 calls, so each lane draws at most one call per round.
@@ -73,6 +74,11 @@ __global__ __launch_bounds__(64) void walk_moments_wide(const uint32_t* __restri
   constexpr int RB = Tl::RB, CB = Tl::CB, NCB = Tl::NCB;
   __shared__ __align__(16) double ys[kRoundMax][2 * DP];  // per step of the round: b, then e
   __shared__ double wz[kRoundMax * kPadDim];              // the round's increments z sqrt(dt), t-major
+  // u_t and u_t^2 of the round's steps, formed once per round by spare lanes
+  // at DP = 32 (3% faster there; at DP = 16 and 24, whose steps carry fewer
+  // FMAs, the extra LDS read per step costs more than the 3 VALU it saves)
+  constexpr bool kUTable = DP == 32;
+  __shared__ __align__(16) double us[kUTable ? kRoundMax : 1][2];
   const int l = threadIdx.x;
   const size_t walk = blockIdx.x;
   if (walk >= n_walks) return;  // whole wave exits together
@@ -130,6 +136,13 @@ __global__ __launch_bounds__(64) void walk_moments_wide(const uint32_t* __restri
         ys[s][DP + l] = bn - b;
         b = bn;
       }
+    } else if (kUTable && l >= kPadDim && l - kPadDim < ns) {
+      // lanes 32.. (never a component's): u_t = (t+1)/T and u_t^2 of step
+      // l - 32, the same for every lane of the moment sums below
+      const int s = l - kPadDim;
+      const double u = (double)(t0 + (uint32_t)s + 1u) * dt;
+      us[s][0] = u;
+      us[s][1] = u * u;
     }
     __syncthreads();
     for (int s = 0; s < ns; ++s) {
@@ -142,11 +155,18 @@ __global__ __launch_bounds__(64) void walk_moments_wide(const uint32_t* __restri
       for (int r = 0; r < RB; ++r)
 #pragma unroll
         for (int c = 0; c < CB; ++c) g[r][c] = fma(y[r], bb[c], g[r][c]);
-      const double u = (double)(t0 + (uint32_t)s + 1u) * dt;
+      double u, u2;
+      if constexpr (kUTable) {
+        u = us[s][0];
+        u2 = us[s][1];
+      } else {
+        u = (double)(t0 + (uint32_t)s + 1u) * dt;
+        u2 = u * u;
+      }
       const double yl = ys[s][lm];
       m0 += yl;
       m1 = fma(u, yl, m1);
-      m2 = fma(u * u, yl, m2);
+      m2 = fma(u2, yl, m2);
     }
     __syncthreads();
   }
