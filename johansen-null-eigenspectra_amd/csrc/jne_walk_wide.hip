@@ -456,8 +456,12 @@ __global__ __launch_bounds__(64) void walk_eigen_wide(int model, int D, const do
   if (l == D - 1) Eg[D - 2] = sub;
   __syncthreads();
 
-  // ---- eigenvalue l (ascending) of the tridiagonal matrix by lane l, by
-  // bisection on Sturm counts from the Gershgorin interval
+  // ---- eigenvalue g (ascending) of the tridiagonal matrix by the P = 64 / D
+  // lanes g, g + D, .., g + (P - 1) D: from the Gershgorin interval, each round
+  // lane g + jD takes the Sturm count at the (j+1)-th of P equally spaced
+  // interior points, the counts meet in LDS, and every lane of the group keeps
+  // the sub-interval whose upper point is the first count past g: (P + 1)-fold
+  // per round, where bisection halves (P = 2 at D = 22..32, 4 at D <= 16).
   double dg[DP], e2[DP];
 #pragma unroll
   for (int j = 0; j < DP; ++j) {
@@ -467,8 +471,36 @@ __global__ __launch_bounds__(64) void walk_eigen_wide(int model, int D, const do
   }
   double lo, hi, pivmin;
   sturm_bounds<DP>(dg, e2, D, lo, hi, pivmin);
-  double lam = 0.0;
-  if (l < D) lam = sturm_eigval<DP, WideMath>(dg, e2, D, l, lo, hi, pivmin, kBisectIters);
+  const int P = 64 / D;  // >= 2 (D <= 32)
+  const int grp = l % D, jp = l / D;  // lanes with jp >= P follow their group
+  const double inv_sections = 1.0 / (double)(P + 1);
+  int rounds = 0;  // (P + 1)^rounds >= 2^54: at most the width 54 halvings leave
+  for (double w = 1.0; w > 0x1p-54; w *= inv_sections) ++rounds;
+  int* const cnt = reinterpret_cast<int*>(Xs);  // 64 ints over Xs, Ps (not read again)
+  for (int it = 0; it < rounds; ++it) {
+    const double step = (hi - lo) * inv_sections;
+    int c = 0;
+    if (jp < P) c = sturm_count<DP, WideMath>(dg, e2, D, lo + (double)(jp + 1) * step, pivmin);
+    cnt[l] = c;
+    __syncthreads();
+    double nlo = lo, nhi = hi;
+    bool found = false;
+    for (int q = 0; q < P; ++q) {
+      const double xq = lo + (double)(q + 1) * step;
+      if (!found) {
+        if (cnt[grp + D * q] > grp) {
+          nhi = xq;
+          found = true;
+        } else {
+          nlo = xq;
+        }
+      }
+    }
+    lo = nlo;
+    hi = nhi;
+    __syncthreads();  // every lane has read the counts before the next round writes them
+  }
+  const double lam = 0.5 * (lo + hi);
 
   // |lambda| sorted descending (rank of each by lane), NX - D structural zeros
   if (l < D) EVs[l] = fabs(lam);
