@@ -3,9 +3,10 @@
 // tridiag_eigvals: a matrix held in one thread's registers, by Householder
 // reduction to tridiagonal form, then the implicit QL iteration with the shift
 // from the leading 2x2 block (the tql1 scheme of the EISPACK/Numerical Recipes
-// family), eigenvalues only.  sturm_*: any one eigenvalue of a tridiagonal
-// matrix by bisection on Sturm counts, so that the lanes of a wave can each
-// find one.
+// family), eigenvalues only.  sturm_*: the Gershgorin interval and the Sturm
+// count of a tridiagonal matrix, from which the lanes of a wave find its
+// eigenvalues one each (or a few lanes per eigenvalue); sturm_eigval is the
+// plain bisection on them.
 //
 // Every array index is a compile-time constant once the loops are unrolled,
 // so the matrix stays in registers: the QL sweep runs over the whole trailing
@@ -198,7 +199,9 @@ JNE_TRI_HD int sturm_count(const double* d, const double* e2, int n, double x, d
   return c;
 }
 
-// Eigenvalue k (0-based, ascending) of T by `iters` halvings of [lo, hi].
+// Eigenvalue k (0-based, ascending) of T by `iters` halvings of [lo, hi] (the
+// one-lane form; the wave-parallel solve runs the same counts at several
+// points per round).
 // From the Gershgorin interval (width <= 2 |T|), 54 halvings leave a width of
 // at most 2^-53 |T|.
 template <int NMAX, class Math>

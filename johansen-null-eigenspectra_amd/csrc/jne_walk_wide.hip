@@ -21,8 +21,8 @@
 //                      with lanes working on matrix entries, rows or columns
 //                      in parallel; then the Householder reduction of M to
 //                      tridiagonal form (lane i holding row i in registers)
-//                      and eigenvalue k by lane k, by bisection on Sturm
-//                      counts (jne_tridiag.h).
+//                      and eigenvalue k by a group of lanes, by multisection
+//                      on Sturm counts (jne_tridiag.h).
 //
 // The arithmetic is FP64 throughout, as in walk_moments_fp64 /
 // walk_eigen_fp64, so the results meet the same 1e-9 parity with the
